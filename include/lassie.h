@@ -1,0 +1,134 @@
+/*
+ * lassie.h -- C ABI of liblassie, an MI355X (gfx950) implementation of LASSIE's
+ * deterministic mass-action integrator.
+ *
+ * The reference has no library interface: its boundary is the process
+ * `LASSIE input_folder output_folder [-v1|-v2|-v3] [-p double|float]`
+ * (src/c++/simulator.cu:29-319) plus the internal seam
+ * `float Simulation<T>::run(...)` (src/c++/header/Simulation.cuh:230-234),
+ * whose inputs are the parsed BioSimWare files.  This header exposes that seam
+ * as plain C: a model is the parsed, sparse-compiled input
+ * (Simulation.cuh:397-418), a solve is the integration loop
+ * (Simulation.cuh:632-925) writing the sampled trajectory (`dinamic`,
+ * Simulation.cuh:500-533, 942).
+ *
+ * Conventions: the caller owns every host array; the library copies what it
+ * needs and owns all device memory inside a model handle.  No function calls
+ * exit(); errors are status codes and lassie_last_error() (thread-local text).
+ * A handle is not thread-safe.
+ */
+#ifndef LASSIE_H
+#define LASSIE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LASSIE_ABI_VERSION 1
+
+/* Status codes.  The CLI maps them to the reference's exit codes
+ * (simulator.cu:61-74, 192-207; Simulation.cuh:38-44, 373-389). */
+#define LASSIE_OK 0
+#define LASSIE_ERR_INVALID 1   /* bad argument or inconsistent model */
+#define LASSIE_ERR_HIP 2       /* HIP runtime failure (reference: exit -13) */
+#define LASSIE_ERR_NOMEM 3
+#define LASSIE_ERR_MAXITER 4   /* opts.max_iters loop iterations reached */
+#define LASSIE_ERR_UNSUPPORTED 5
+#define LASSIE_ERR_DEVICE 6    /* a device-side wait exceeded its bound */
+#define LASSIE_ERR_IO 7        /* file could not be read or written */
+
+/* Arithmetic of the Newton linear solve. */
+#define LASSIE_PREC_FP64 0          /* plain method: FP64 LU (cuSOLVER getrf/getrs seat) */
+#define LASSIE_PREC_FP32_REFINE 1   /* FP32 LU + FP64 iterative refinement */
+#define LASSIE_PREC_BF16_REFINE 2   /* BF16 LU + FP64 iterative refinement */
+
+typedef struct lassie_model lassie_model;
+
+/* A compiled model (Simulation.cuh:397-418 builds the same data as short4 /
+ * short2 arrays; here int32 CSR).
+ *   reactants, reaction-major, species ascending (create_sparse2,
+ *     Simulation.cuh:1107-1130): react_ptr[R+1], react_species[], react_exp[]
+ *   net stoichiometry H = (right - left)^T, species-major, reaction ascending
+ *     (createTransposte + create_sparse, Simulation.cuh:1061-1105):
+ *     net_ptr[S+1], net_reaction[], net_coef[]
+ *   k[R] (c_vector), x0[S] (M_0), feed[S] (M_feed, NULL = all zero),
+ *   atol[S] (atol_vector, NULL = 1e-12 each, Simulation.cuh:317-325). */
+typedef struct {
+    int32_t n_species;
+    int32_t n_reactions;
+    const int32_t *react_ptr;
+    const int32_t *react_species;
+    const int32_t *react_exp;
+    const int32_t *net_ptr;
+    const int32_t *net_reaction;
+    const int32_t *net_coef;
+    const double *k;
+    const double *x0;
+    const double *feed;
+    const double *atol;
+} lassie_model_desc;
+
+/* Integrator options; defaults (lassie_opts_default) are the reference's
+ * (Simulation.cuh:271-354). */
+typedef struct {
+    double rkf_step;       /* initial RKF45 step, file rkf_step, default 1e-3 */
+    double be_step;        /* BDF1 step, file be_step, default 1e-2 */
+    double newton_tol;     /* file newton_tol, default 1e-6 */
+    double stiffness_tol;  /* file stiffness_tol, default 1e-6 */
+    double rtol;           /* extension: tol = atol + rtol*|w1| when > 0; default 0 */
+    int32_t newton_iter;   /* file newton_iter, default 10000 */
+    int32_t bdf_regrow;    /* extension: BDF step doubles back to be_step; default 0 */
+    int32_t max_rkf_retries; /* extension: N rejections in a row force BDF; default 0 = off */
+    int32_t precision;     /* LASSIE_PREC_*, default LASSIE_PREC_FP64 */
+    int64_t max_iters;     /* cap on loop iterations; default 100000000 */
+    double refine_tol;     /* refinement stop on ||r||_inf / ||b||_inf, default 1e-10 */
+    int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback, default 8 */
+    int32_t reserved;
+} lassie_opts;
+
+typedef struct {
+    int64_t loop_iters, rkf_attempts, rkf_accepts, bdf_steps, newton_iters;
+    int64_t lu_failures, nan_repairs, samples;
+    double t_final, dt_final, newdt_final;
+    double kernel_ms;      /* device time of the integration loop */
+    int64_t lu_factorizations, refine_iters, refine_fallbacks;
+} lassie_stats;
+
+void lassie_opts_default(lassie_opts *opts);
+
+/* Copies the model to device memory.  Reference: Simulation.cuh:470-576. */
+int lassie_model_create(const lassie_model_desc *desc, lassie_model **out);
+int lassie_model_destroy(lassie_model *model);
+
+/* Integrates from t = 0 (Simulation.cuh:632-636) and stores the state of
+ * species cs[j] at every sampling time t[i] into out[i*ncs + j]
+ * (saveDinamic, kernels.cuh:524-535).  y_final[S] (may be NULL) receives the
+ * state when the loop ends; progress (may be NULL, capacity progress_cap)
+ * receives t/t_max at the top of every loop iteration (ProgressSimulation,
+ * Simulation.cuh:660-672).  Reference: Simulation<T>::run, Simulation.cuh:230-981. */
+int lassie_solve(lassie_model *model, const lassie_opts *opts, const double *t, int32_t nt,
+                 const int32_t *cs, int32_t ncs, double *out, double *y_final,
+                 double *progress, int64_t progress_cap, lassie_stats *stats);
+
+/* Host helper: CSR arrays from dense R x S stoichiometry (int16, row-major, as
+ * Reader<T>::readMatrix returns them, Reader.h:104-124), in the entry order
+ * of create_sparse2 / create_sparse (Simulation.cuh:1079-1130).  Output
+ * arrays must hold R+1 / nnz(left) / S+1 / nnz(H) entries; pass NULL outputs
+ * to query the two counts first. */
+int lassie_csr_from_dense(int32_t n_species, int32_t n_reactions, const int16_t *left,
+                          const int16_t *right, int32_t *nnz_left, int32_t *nnz_net,
+                          int32_t *react_ptr, int32_t *react_species, int32_t *react_exp,
+                          int32_t *net_ptr, int32_t *net_reaction, int32_t *net_coef);
+
+/* Number of visible devices and the name of device 0 (name may be NULL). */
+int lassie_device_count(int32_t *count);
+
+const char *lassie_last_error(void);
+int lassie_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,193 @@
+"""ctypes binding of liblassie.so (the C ABI declared in include/lassie.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C lassie_amd/csrc``).  Nothing here falls back to another
+implementation: if the library is missing, or no HIP device is visible when a
+model is created, the call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblassie.so")
+
+LASSIE_OK = 0
+LASSIE_ERR_INVALID = 1
+LASSIE_ERR_HIP = 2
+LASSIE_ERR_NOMEM = 3
+LASSIE_ERR_MAXITER = 4
+LASSIE_ERR_UNSUPPORTED = 5
+LASSIE_ERR_DEVICE = 6
+LASSIE_ERR_IO = 7
+
+PREC_FP64 = 0
+PREC_FP32_REFINE = 1
+PREC_BF16_REFINE = 2
+
+_i32p = C.POINTER(C.c_int32)
+_i16p = C.POINTER(C.c_int16)
+_f64p = C.POINTER(C.c_double)
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("n_species", C.c_int32), ("n_reactions", C.c_int32),
+                ("react_ptr", _i32p), ("react_species", _i32p), ("react_exp", _i32p),
+                ("net_ptr", _i32p), ("net_reaction", _i32p), ("net_coef", _i32p),
+                ("k", _f64p), ("x0", _f64p), ("feed", _f64p), ("atol", _f64p)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("rkf_step", C.c_double), ("be_step", C.c_double), ("newton_tol", C.c_double),
+                ("stiffness_tol", C.c_double), ("rtol", C.c_double), ("newton_iter", C.c_int32),
+                ("bdf_regrow", C.c_int32), ("max_rkf_retries", C.c_int32),
+                ("precision", C.c_int32), ("max_iters", C.c_int64), ("refine_tol", C.c_double),
+                ("refine_max_iter", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("loop_iters", C.c_int64), ("rkf_attempts", C.c_int64),
+                ("rkf_accepts", C.c_int64), ("bdf_steps", C.c_int64),
+                ("newton_iters", C.c_int64), ("lu_failures", C.c_int64),
+                ("nan_repairs", C.c_int64), ("samples", C.c_int64),
+                ("t_final", C.c_double), ("dt_final", C.c_double), ("newdt_final", C.c_double),
+                ("kernel_ms", C.c_double), ("lu_factorizations", C.c_int64),
+                ("refine_iters", C.c_int64), ("refine_fallbacks", C.c_int64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+# every symbol include/lassie.h declares
+EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy", "lassie_solve",
+           "lassie_csr_from_dense", "lassie_device_count", "lassie_last_error",
+           "lassie_abi_version")
+
+
+class LassieError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"liblassie error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                               "or make -C lassie_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        L.lassie_opts_default.argtypes = [C.POINTER(Opts)]
+        L.lassie_opts_default.restype = None
+        L.lassie_model_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(C.c_void_p)]
+        L.lassie_model_destroy.argtypes = [C.c_void_p]
+        L.lassie_solve.argtypes = [C.c_void_p, C.POINTER(Opts), _f64p, C.c_int32, _i32p, C.c_int32,
+                                   _f64p, _f64p, _f64p, C.c_int64, C.POINTER(Stats)]
+        L.lassie_csr_from_dense.argtypes = [C.c_int32, C.c_int32, _i16p, _i16p, _i32p, _i32p,
+                                            _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]
+        L.lassie_device_count.argtypes = [_i32p]
+        L.lassie_last_error.restype = C.c_char_p
+        L.lassie_abi_version.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != LASSIE_OK:
+        raise LassieError(rc, lib().lassie_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().lassie_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    rc = lib().lassie_device_count(C.byref(n))
+    return n.value if rc == LASSIE_OK else 0
+
+
+def csr_from_dense(L: np.ndarray, Rm: np.ndarray):
+    """The library's own dense -> CSR build (create_sparse / create_sparse2)."""
+    L = np.ascontiguousarray(L, np.int16)
+    Rm = np.ascontiguousarray(Rm, np.int16)
+    R, S = L.shape
+    nl, nh = C.c_int32(0), C.c_int32(0)
+    _check(lib().lassie_csr_from_dense(S, R, _p(L, _i16p), _p(Rm, _i16p), C.byref(nl), C.byref(nh),
+                                       None, None, None, None, None, None))
+    out = [np.zeros(R + 1, np.int32), np.zeros(max(nl.value, 1), np.int32),
+           np.zeros(max(nl.value, 1), np.int32), np.zeros(S + 1, np.int32),
+           np.zeros(max(nh.value, 1), np.int32), np.zeros(max(nh.value, 1), np.int32)]
+    _check(lib().lassie_csr_from_dense(S, R, _p(L, _i16p), _p(Rm, _i16p), None, None,
+                                       *[_p(a, _i32p) for a in out]))
+    lptr, lsp, lexp, hptr, hrx, hcoef = out
+    return lptr, lsp[:nl.value], lexp[:nl.value], hptr, hrx[:nh.value], hcoef[:nh.value]
+
+
+class DeviceModel:
+    """A model resident in device memory (lassie_model_create)."""
+
+    def __init__(self, m):
+        self.S, self.R = m.S, m.R
+        keep = dict(
+            lptr=np.ascontiguousarray(m.lptr, np.int32), lsp=np.ascontiguousarray(m.lsp, np.int32),
+            lexp=np.ascontiguousarray(m.lexp, np.int32), hptr=np.ascontiguousarray(m.hptr, np.int32),
+            hrx=np.ascontiguousarray(m.hrx, np.int32), hcoef=np.ascontiguousarray(m.hcoef, np.int32),
+            k=np.ascontiguousarray(m.k, np.float64), x0=np.ascontiguousarray(m.x0, np.float64),
+            feed=np.ascontiguousarray(m.feed, np.float64),
+            atol=np.ascontiguousarray(m.atol, np.float64))
+        for key in ("lsp", "lexp", "hrx", "hcoef"):
+            if keep[key].size == 0:
+                keep[key] = np.zeros(1, np.int32)
+        d = ModelDesc(m.S, m.R, _p(keep["lptr"], _i32p), _p(keep["lsp"], _i32p),
+                      _p(keep["lexp"], _i32p), _p(keep["hptr"], _i32p), _p(keep["hrx"], _i32p),
+                      _p(keep["hcoef"], _i32p), _p(keep["k"], _f64p), _p(keep["x0"], _f64p),
+                      _p(keep["feed"], _f64p), _p(keep["atol"], _f64p))
+        h = C.c_void_p()
+        _check(lib().lassie_model_create(C.byref(d), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lassie_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, t, cs=None, progress_cap=0, raise_on_maxiter=True, **opts):
+        """Returns (out[nt, ncs], y_final[S], stats dict, status, progress)."""
+        t = np.ascontiguousarray(t, np.float64)
+        cs = np.arange(self.S, dtype=np.int32) if cs is None else np.ascontiguousarray(cs, np.int32)
+        out = np.full((t.size, cs.size), np.nan)
+        yf = np.zeros(self.S)
+        prog = np.zeros(max(progress_cap, 1))
+        st = Stats()
+        o = default_opts(**opts)
+        rc = lib().lassie_solve(self.h, C.byref(o), _p(t, _f64p), t.size, _p(cs, _i32p), cs.size,
+                                _p(out, _f64p), _p(yf, _f64p),
+                                _p(prog, _f64p) if progress_cap else None, progress_cap,
+                                C.byref(st))
+        if rc != LASSIE_OK and not (rc == LASSIE_ERR_MAXITER and not raise_on_maxiter):
+            _check(rc)
+        d = st.as_dict()
+        return out, yf, d, rc, prog[:min(progress_cap, d["loop_iters"])]

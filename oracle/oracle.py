@@ -1,0 +1,184 @@
+"""ctypes front end of the oracle (``lassie_oracle.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py``, as the checker.  The product never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liblassie_oracle.so")
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class OrcModel(C.Structure):
+    _fields_ = [("S", C.c_int32), ("R", C.c_int32),
+                ("lptr", _i32p), ("lsp", _i32p), ("lexp", _i32p),
+                ("hptr", _i32p), ("hrx", _i32p), ("hcoef", _i32p),
+                ("k", _f64p), ("x0", _f64p), ("feed", _f64p), ("atol", _f64p)]
+
+
+class OrcOpts(C.Structure):
+    _fields_ = [("rkf_step", C.c_double), ("be_step", C.c_double), ("newton_tol", C.c_double),
+                ("stiffness_tol", C.c_double), ("rtol", C.c_double), ("newton_iter", C.c_int32),
+                ("bdf_regrow", C.c_int32), ("max_rkf_retries", C.c_int32), ("pad_", C.c_int32),
+                ("max_iters", C.c_int64)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("loop_iters", C.c_int64), ("rkf_attempts", C.c_int64),
+                ("rkf_accepts", C.c_int64), ("bdf_steps", C.c_int64),
+                ("newton_iters", C.c_int64), ("lu_failures", C.c_int64),
+                ("nan_repairs", C.c_int64), ("samples", C.c_int64),
+                ("t_final", C.c_double), ("dt_final", C.c_double), ("newdt_final", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.orc_rhs.argtypes = [C.POINTER(OrcModel), _f64p, _f64p]
+        L.orc_jac.argtypes = [C.POINTER(OrcModel), _f64p, _f64p]
+        L.orc_newton_matrix.argtypes = [C.c_int, C.c_double, _f64p]
+        L.orc_lu.argtypes = [C.c_int, _f64p, _i32p]
+        L.orc_lu.restype = C.c_int
+        L.orc_lu_solve.argtypes = [C.c_int, _f64p, _i32p, _f64p]
+        L.orc_normsq.argtypes = [C.c_int, _f64p]
+        L.orc_normsq.restype = C.c_double
+        L.orc_min_first.argtypes = [C.c_int, _f64p]
+        L.orc_min_first.restype = C.c_double
+        L.orc_rkf_attempt.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_double, _f64p, _f64p,
+                                      _f64p, _f64p]
+        L.orc_rkf_attempt.restype = C.c_int
+        L.orc_solve.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcOpts), _f64p, C.c_int, _i32p,
+                                C.c_int, _f64p, _f64p, _f64p, C.c_int64, C.POINTER(OrcStats)]
+        L.orc_solve.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class _Bound:
+    """Keeps the numpy arrays alive while the C struct points at them."""
+
+    def __init__(self, m):
+        self.arrs = dict(
+            lptr=np.ascontiguousarray(m.lptr, np.int32), lsp=np.ascontiguousarray(m.lsp, np.int32),
+            lexp=np.ascontiguousarray(m.lexp, np.int32), hptr=np.ascontiguousarray(m.hptr, np.int32),
+            hrx=np.ascontiguousarray(m.hrx, np.int32), hcoef=np.ascontiguousarray(m.hcoef, np.int32),
+            k=np.ascontiguousarray(m.k, np.float64), x0=np.ascontiguousarray(m.x0, np.float64),
+            feed=np.ascontiguousarray(m.feed, np.float64),
+            atol=np.ascontiguousarray(m.atol, np.float64))
+        a = self.arrs
+        # zero-length arrays still need a valid pointer
+        for key in ("lsp", "lexp", "hrx", "hcoef"):
+            if a[key].size == 0:
+                a[key] = np.zeros(1, np.int32)
+        self.s = OrcModel(m.S, m.R, _p(a["lptr"], _i32p), _p(a["lsp"], _i32p),
+                          _p(a["lexp"], _i32p), _p(a["hptr"], _i32p), _p(a["hrx"], _i32p),
+                          _p(a["hcoef"], _i32p), _p(a["k"], _f64p), _p(a["x0"], _f64p),
+                          _p(a["feed"], _f64p), _p(a["atol"], _f64p))
+
+
+def opts(rkf_step=1e-3, be_step=1e-2, newton_tol=1e-6, stiffness_tol=1e-6, rtol=0.0,
+         newton_iter=10000, bdf_regrow=0, max_rkf_retries=0, max_iters=0):
+    return OrcOpts(rkf_step, be_step, newton_tol, stiffness_tol, rtol, newton_iter, bdf_regrow,
+                   max_rkf_retries, 0, max_iters)
+
+
+def rhs(m, x):
+    b = _Bound(m)
+    x = np.ascontiguousarray(x, np.float64)
+    f = np.empty(m.S)
+    lib().orc_rhs(C.byref(b.s), _p(x, _f64p), _p(f, _f64p))
+    return f
+
+
+def jac(m, x):
+    b = _Bound(m)
+    x = np.ascontiguousarray(x, np.float64)
+    J = np.empty((m.S, m.S))
+    lib().orc_jac(C.byref(b.s), _p(x, _f64p), _p(J, _f64p))
+    return J
+
+
+def newton_matrix(J, h):
+    M = np.array(J, dtype=np.float64, order="C", copy=True)
+    lib().orc_newton_matrix(M.shape[0], h, _p(M, _f64p))
+    return M
+
+
+def lu(A):
+    LU = np.array(A, dtype=np.float64, order="C", copy=True)
+    n = LU.shape[0]
+    piv = np.zeros(n, np.int32)
+    info = lib().orc_lu(n, _p(LU, _f64p), _p(piv, _i32p))
+    return LU, piv, info
+
+
+def lu_solve(LU, piv, b):
+    x = np.array(b, dtype=np.float64, copy=True)
+    LU = np.ascontiguousarray(LU, np.float64)
+    piv = np.ascontiguousarray(piv, np.int32)
+    lib().orc_lu_solve(LU.shape[0], _p(LU, _f64p), _p(piv, _i32p), _p(x, _f64p))
+    return x
+
+
+def normsq(d):
+    d = np.ascontiguousarray(d, np.float64)
+    return lib().orc_normsq(d.size, _p(d, _f64p))
+
+
+def min_first(d):
+    d = np.ascontiguousarray(d, np.float64)
+    return lib().orc_min_first(d.size, _p(d, _f64p))
+
+
+def rkf_attempt(m, y1, dt, rtol=0.0):
+    b = _Bound(m)
+    y1 = np.ascontiguousarray(y1, np.float64)
+    w1, w2, delta = np.empty(m.S), np.empty(m.S), np.empty(m.S)
+    nok = lib().orc_rkf_attempt(C.byref(b.s), rtol, dt, _p(y1, _f64p), _p(w1, _f64p),
+                                _p(w2, _f64p), _p(delta, _f64p))
+    return nok, w1, w2, delta
+
+
+def solve(m, t, cs=None, progress_cap=0, **kw):
+    """Full trajectory; returns (out[nt, ncs], stats dict, rc, progress, y_final)."""
+    b = _Bound(m)
+    t = np.ascontiguousarray(t, np.float64)
+    cs = np.arange(m.S, dtype=np.int32) if cs is None else np.ascontiguousarray(cs, np.int32)
+    out = np.full((t.size, cs.size), np.nan)
+    o = opts(**kw)
+    st = OrcStats()
+    prog = np.zeros(max(progress_cap, 1))
+    yf = np.zeros(m.S)
+    rc = lib().orc_solve(C.byref(b.s), C.byref(o), _p(t, _f64p), t.size, _p(cs, _i32p), cs.size,
+                         _p(out, _f64p), _p(yf, _f64p),
+                         _p(prog, _f64p) if progress_cap else None, progress_cap, C.byref(st))
+    d = st.as_dict()
+    n_prog = min(progress_cap, d["loop_iters"])
+    return out, d, rc, prog[:n_prog], yf
