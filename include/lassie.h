@@ -94,6 +94,7 @@ typedef struct {
     double t_final, dt_final, newdt_final;
     double kernel_ms;      /* device time of the integration loop */
     int64_t lu_factorizations, refine_iters, refine_fallbacks;
+    double phase_ms[4];    /* device time by phase: control+RKF45, Jacobian, LU, Newton */
 } lassie_stats;
 
 void lassie_opts_default(lassie_opts *opts);

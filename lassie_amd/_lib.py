@@ -55,10 +55,13 @@ class Stats(C.Structure):
                 ("nan_repairs", C.c_int64), ("samples", C.c_int64),
                 ("t_final", C.c_double), ("dt_final", C.c_double), ("newdt_final", C.c_double),
                 ("kernel_ms", C.c_double), ("lu_factorizations", C.c_int64),
-                ("refine_iters", C.c_int64), ("refine_fallbacks", C.c_int64)]
+                ("refine_iters", C.c_int64), ("refine_fallbacks", C.c_int64),
+                ("phase_ms", C.c_double * 4)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["phase_ms"] = list(self.phase_ms)
+        return d
 
 
 # every symbol include/lassie.h declares

@@ -367,7 +367,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     long long st[16];
     double dst[8];
     HIP_TRY(hipMemcpyAsync(st, m->d_stats, sizeof(st), hipMemcpyDeviceToHost, m->stream));
-    HIP_TRY(hipMemcpyAsync(dst, m->d_dstats, sizeof(double) * 3, hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipMemcpyAsync(dst, m->d_dstats, sizeof(double) * 7, hipMemcpyDeviceToHost, m->stream));
     lassie::Ctl ctl;
     HIP_TRY(hipMemcpyAsync(&ctl, m->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, m->stream));
     if (ncs) HIP_TRY(hipMemcpyAsync(out, m->d_out, sizeof(double) * (size_t)nt * ncs, hipMemcpyDeviceToHost, m->stream));
@@ -396,6 +396,10 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
         stats->dt_final = dst[1];
         stats->newdt_final = dst[2];
         stats->kernel_ms = ms;
+        stats->phase_ms[0] = dst[3];
+        stats->phase_ms[1] = dst[4];
+        stats->phase_ms[2] = dst[5];
+        stats->phase_ms[3] = dst[6];
     }
     if (st[9]) return fail(LASSIE_ERR_MAXITER, "max_iters reached before the last sample");
     return LASSIE_OK;

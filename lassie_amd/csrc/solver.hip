@@ -339,9 +339,6 @@ __device__ void panel_factor(double *__restrict__ M, int n, int p, int *piv, int
                 pos[q] = bp;
             }
         }
-        double pr[NBC];
-#pragma unroll
-        for (int c = 0; c < NBC; c++) pr[c] = (c > j) ? prow[c] : 0.0;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
             if (act[q]) {
@@ -349,7 +346,7 @@ __device__ void panel_factor(double *__restrict__ M, int n, int p, int *piv, int
                 a[q][j] = l;
 #pragma unroll
                 for (int c = 0; c < NBC; c++)
-                    if (c > j) a[q][c] = fma(-l, pr[c], a[q][c]);
+                    if (c > j) a[q][c] = fma(-l, prow[c], a[q][c]);
             }
         }
         buf ^= 1;
@@ -413,21 +410,33 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
         const int kk = e / NBC, c = e % NBC;
         if (kk < nbp && c < ncols) M[(size_t)(c0 + c) * n + r0 + kk] = sh.U12[kk][c];
     }
-    // trailing update A22 -= L21 * U12, fma chain with k ascending
-    for (int i = r0 + nbp + tid; i < n; i += NT) {
-        double l[NBC];
+    // trailing update A22 -= L21 * U12, fma chain with k ascending.  A wave
+    // walks 64 consecutive rows (coalesced in the column-major layout) and
+    // owns CPW columns of the strip, whose U12 entries stay in registers.
+    constexpr int CPW = NBC / NWAVES;
+    const int lane = tid & 63;
+    const int cbase = (tid >> 6) * CPW;
+    double u[NBC][CPW];
 #pragma unroll
-        for (int kk = 0; kk < NBC; kk++) l[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + i] : 0.0;
+    for (int kk = 0; kk < NBC; kk++)
 #pragma unroll
-        for (int c = 0; c < NBC; c++) {
-            if (c < ncols) {
-                double acc = M[(size_t)(c0 + c) * n + i];
+        for (int cc = 0; cc < CPW; cc++) u[kk][cc] = (kk < nbp) ? sh.U12[kk][cbase + cc] : 0.0;
+    for (int i = r0 + nbp + lane; i < n; i += 64) {
+        double acc[CPW];
 #pragma unroll
-                for (int kk = 0; kk < NBC; kk++)
-                    if (kk < nbp) acc = fma(-l[kk], sh.U12[kk][c], acc);
-                M[(size_t)(c0 + c) * n + i] = acc;
+        for (int cc = 0; cc < CPW; cc++)
+            acc[cc] = (cbase + cc < ncols) ? M[(size_t)(c0 + cbase + cc) * n + i] : 0.0;
+#pragma unroll
+        for (int kk = 0; kk < NBC; kk++) {
+            if (kk < nbp) {
+                const double l = M[(size_t)(r0 + kk) * n + i];
+#pragma unroll
+                for (int cc = 0; cc < CPW; cc++) acc[cc] = fma(-l, u[kk][cc], acc[cc]);
             }
         }
+#pragma unroll
+        for (int cc = 0; cc < CPW; cc++)
+            if (cbase + cc < ncols) M[(size_t)(c0 + cbase + cc) * n + i] = acc[cc];
     }
     __syncthreads();
 }
@@ -569,6 +578,10 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     long long st_loop = 0, st_att = 0, st_acc = 0, st_bdf = 0, st_newton = 0, st_lufail = 0, st_rep = 0,
               st_lu = 0;
     int rc = 0;
+    // device-time attribution (group 0, 100 MHz wall clock): control/RKF,
+    // Jacobian assembly, LU, Newton solves
+    unsigned long long ph[4] = {0, 0, 0, 0};
+    unsigned long long tmark = 0;
 
     if (wg == 0)
         for (int s = tid; s < S; s += NT) A.y1[s] = A.x0[s];
@@ -577,6 +590,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     while (true) {
         // ---- control step (group 0) -----------------------------------------
         if (wg == 0) {
+            tmark = wall_clock64();
             int mode = MODE_CONTINUE;
             double hb = 0.0;
             if (A.max_iters > 0 && st_loop >= A.max_iters) {
@@ -642,6 +656,11 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 st_rel(&ctl->info, 0);
             }
         }
+        if (wg == 0) {
+            const unsigned long long now = wall_clock64();
+            ph[0] += now - tmark;
+            tmark = now;
+        }
         if (G > 1) {
             if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
         } else {
@@ -671,7 +690,17 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             } else {
                 __syncthreads();
             }
+            if (wg == 0) {
+                const unsigned long long now = wall_clock64();
+                ph[1] += now - tmark;
+                tmark = now;
+            }
             if (!lu_factor_grid<RPT, NBC>(A, sh, gen_local, singular)) return;
+            if (wg == 0) {
+                const unsigned long long now = wall_clock64();
+                ph[2] += now - tmark;
+                tmark = now;
+            }
         }
         // ---- Newton loop, group 0 --------------------------------------------
         if (wg == 0) {
@@ -732,6 +761,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             for (int s = tid; s < S; s += NT) A.y1[s] = A.yv[s];
             __syncthreads();
             t = t + h;
+            ph[3] += wall_clock64() - tmark;
         }
     }
     if (wg == 0 && tid == 0) {
@@ -750,6 +780,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         d[0] = t;
         d[1] = dt;
         d[2] = newdt;
+        for (int q = 0; q < 4; q++) d[3 + q] = (double)ph[q] * 1e-5;  // 100 MHz ticks -> ms
     }
 }
 
