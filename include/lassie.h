@@ -85,7 +85,7 @@ typedef struct {
     int64_t max_iters;     /* cap on loop iterations; default 100000000 */
     double refine_tol;     /* refinement stop on ||r||_inf / ||b||_inf, default 1e-10 */
     int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback, default 8 */
-    int32_t reserved;
+    int32_t reserved;      /* diagnostics: bit 1 keeps the control group's state in global memory */
 } lassie_opts;
 
 typedef struct {
@@ -95,6 +95,8 @@ typedef struct {
     double kernel_ms;      /* device time of the integration loop */
     int64_t lu_factorizations, refine_iters, refine_fallbacks;
     double phase_ms[4];    /* device time by phase: control+RKF45, Jacobian, LU, Newton */
+    int32_t groups;        /* workgroups that cooperated on the solve */
+    int32_t lds_model;     /* 1 when the control group kept model and state in LDS */
 } lassie_stats;
 
 void lassie_opts_default(lassie_opts *opts);
@@ -112,6 +114,18 @@ int lassie_model_destroy(lassie_model *model);
 int lassie_solve(lassie_model *model, const lassie_opts *opts, const double *t, int32_t nt,
                  const int32_t *cs, int32_t ncs, double *out, double *y_final,
                  double *progress, int64_t progress_cap, lassie_stats *stats);
+
+/* Batched form: integrates n_models independent models (S <= 512 each) in one
+ * persistent launch, every model with its own step control, all sampled at
+ * the same times t.  out receives, model after model, nt x S_i values (every
+ * species); y_final (may be NULL) the S_i final states, model after model;
+ * stats (may be NULL) one record per model; elapsed_ms (may be NULL) the
+ * device time of the launch.  opts.precision selects the Newton linear solve
+ * (LASSIE_PREC_*).  No counterpart in the reference, which runs one model per
+ * process; each instance follows Simulation<T>::run (Simulation.cuh:632-925). */
+int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                       const double *t, int32_t nt, double *out, double *y_final, lassie_stats *stats,
+                       double *elapsed_ms);
 
 /* Host helper: CSR arrays from dense R x S stoichiometry (int16, row-major, as
  * Reader<T>::readMatrix returns them, Reader.h:104-124), in the entry order

@@ -56,7 +56,7 @@ class Stats(C.Structure):
                 ("t_final", C.c_double), ("dt_final", C.c_double), ("newdt_final", C.c_double),
                 ("kernel_ms", C.c_double), ("lu_factorizations", C.c_int64),
                 ("refine_iters", C.c_int64), ("refine_fallbacks", C.c_int64),
-                ("phase_ms", C.c_double * 4)]
+                ("phase_ms", C.c_double * 4), ("groups", C.c_int32), ("lds_model", C.c_int32)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -66,8 +66,8 @@ class Stats(C.Structure):
 
 # every symbol include/lassie.h declares
 EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy", "lassie_solve",
-           "lassie_csr_from_dense", "lassie_device_count", "lassie_last_error",
-           "lassie_abi_version")
+           "lassie_solve_batch", "lassie_csr_from_dense", "lassie_device_count",
+           "lassie_last_error", "lassie_abi_version")
 
 
 class LassieError(RuntimeError):
@@ -92,6 +92,9 @@ def lib():
         L.lassie_model_destroy.argtypes = [C.c_void_p]
         L.lassie_solve.argtypes = [C.c_void_p, C.POINTER(Opts), _f64p, C.c_int32, _i32p, C.c_int32,
                                    _f64p, _f64p, _f64p, C.c_int64, C.POINTER(Stats)]
+        L.lassie_solve_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts), _f64p,
+                                         C.c_int32, _f64p, _f64p, C.POINTER(Stats),
+                                         C.POINTER(C.c_double)]
         L.lassie_csr_from_dense.argtypes = [C.c_int32, C.c_int32, _i16p, _i16p, _i32p, _i32p,
                                             _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]
         L.lassie_device_count.argtypes = [_i32p]
@@ -194,3 +197,29 @@ class DeviceModel:
             _check(rc)
         d = st.as_dict()
         return out, yf, d, rc, prog[:min(progress_cap, d["loop_iters"])]
+
+
+def solve_batch(models, t, raise_on_maxiter=True, **opts):
+    """Batched form over uploaded DeviceModels.  Returns (list of out[nt, S_i],
+    list of y_final[S_i], list of stats dicts, device ms, status)."""
+    t = np.ascontiguousarray(t, np.float64)
+    n = len(models)
+    sizes = [dm.S for dm in models]
+    out = np.full(t.size * sum(sizes), np.nan)
+    yf = np.zeros(sum(sizes))
+    st = (Stats * n)()
+    hs = (C.c_void_p * n)(*[dm.h for dm in models])
+    ms = C.c_double(0.0)
+    o = default_opts(**opts)
+    rc = lib().lassie_solve_batch(hs, n, C.byref(o), _p(t, _f64p), t.size, _p(out, _f64p),
+                                  _p(yf, _f64p), st, C.byref(ms))
+    if rc != LASSIE_OK and not (rc == LASSIE_ERR_MAXITER and not raise_on_maxiter):
+        _check(rc)
+    outs, ys = [], []
+    o_off = y_off = 0
+    for S in sizes:
+        outs.append(out[o_off:o_off + t.size * S].reshape(t.size, S))
+        ys.append(yf[y_off:y_off + S])
+        o_off += t.size * S
+        y_off += S
+    return outs, ys, [s.as_dict() for s in st], ms.value, rc

@@ -47,9 +47,11 @@ struct lassie_model {
     std::vector<double> k, x0, feed, atol, hk;
     int *d_lptr = nullptr, *d_lsp = nullptr, *d_lexp = nullptr, *d_hptr = nullptr, *d_hrx = nullptr;
     double *d_hk = nullptr, *d_x0 = nullptr, *d_feed = nullptr, *d_atol = nullptr;
+    lassie::HRec *d_recs = nullptr;
+    std::vector<lassie::HRec> recs;
     double *d_vec = nullptr;  // work vectors, 12 x S
     double *d_M = nullptr;    // dense S x S Newton matrix (column-major)
-    int *d_piv = nullptr, *d_perm = nullptr, *d_moved = nullptr;
+    int *d_piv = nullptr, *d_perm = nullptr, *d_moved = nullptr, *d_rows = nullptr, *d_rowcnt = nullptr;
     lassie::Ctl *d_ctl = nullptr;
     lassie::GridSync *d_gs = nullptr;
     long long *d_stats = nullptr;
@@ -64,8 +66,8 @@ struct lassie_model {
 static void free_model(lassie_model *m)
 {
     void *ptrs[] = {m->d_lptr, m->d_lsp, m->d_lexp, m->d_hptr, m->d_hrx, m->d_hk, m->d_x0, m->d_feed,
-                    m->d_atol, m->d_vec, m->d_M, m->d_piv, m->d_perm, m->d_moved, m->d_ctl, m->d_gs,
-                    m->d_stats, m->d_dstats, m->d_tv, m->d_out, m->d_prog, m->d_cs};
+                    m->d_atol, m->d_vec, m->d_M, m->d_piv, m->d_perm, m->d_moved, m->d_rows, m->d_rowcnt, m->d_ctl, m->d_gs,
+                    m->d_stats, m->d_dstats, m->d_tv, m->d_out, m->d_prog, m->d_cs, m->d_recs};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -209,6 +211,29 @@ int lassie_model_create(const lassie_model_desc *d, lassie_model **out)
     if (d->atol) m->atol.assign(d->atol, d->atol + S);
     m->hk.resize(nh);
     for (int e = 0; e < nh; e++) m->hk[e] = (double)m->hcoef[e] * m->k[m->hrx[e]];
+    // net entries with their reactions' reactants inlined (see HRec)
+    m->recs.resize(nh);
+    for (int e = 0; e < nh; e++) {
+        const int r = m->hrx[e];
+        const int a0 = m->lptr[r], na = m->lptr[r + 1] - a0;
+        lassie::HRec rc;
+        rc.hk = m->hk[e];
+        bool inl = na <= 2 && S <= 32767;
+        for (int j = 0; j < na && inl; j++) inl = m->lexp[a0 + j] >= 0 && m->lexp[a0 + j] <= 255;
+        if (na == 0) {
+            rc.sp01 = 0;
+            rc.meta = 0;
+        } else if (inl) {
+            const int s0 = m->lsp[a0], e0 = m->lexp[a0];
+            const int s1 = na == 2 ? m->lsp[a0 + 1] : 0, e1 = na == 2 ? m->lexp[a0 + 1] : 0;
+            rc.sp01 = s0 | (s1 << 16);
+            rc.meta = e0 | (e1 << 8) | (na << 16);
+        } else {
+            rc.sp01 = r;
+            rc.meta = 3 << 16;
+        }
+        m->recs[e] = rc;
+    }
     auto bail = [&](int code) {
         free_model(m);
         return code;
@@ -219,6 +244,7 @@ int lassie_model_create(const lassie_model_desc *d, lassie_model **out)
     if ((rc = dev_upload(&m->d_hptr, m->hptr.data(), m->hptr.size()))) return bail(rc);
     if ((rc = dev_upload(&m->d_hrx, m->hrx.data(), m->hrx.size()))) return bail(rc);
     if ((rc = dev_upload(&m->d_hk, m->hk.data(), m->hk.size()))) return bail(rc);
+    if ((rc = dev_upload(&m->d_recs, m->recs.data(), m->recs.size()))) return bail(rc);
     if ((rc = dev_upload(&m->d_x0, m->x0.data(), m->x0.size()))) return bail(rc);
     if ((rc = dev_upload(&m->d_feed, m->feed.data(), m->feed.size()))) return bail(rc);
     if ((rc = dev_upload(&m->d_atol, m->atol.data(), m->atol.size()))) return bail(rc);
@@ -243,6 +269,12 @@ int lassie_model_create(const lassie_model_desc *d, lassie_model **out)
         return bail(LASSIE_ERR_HIP);
     }
     if (S <= lassie::kMaxDenseN) {
+        const int P = (S + lassie::dense_panel_width(S) - 1) / lassie::dense_panel_width(S);
+        if ((e = hipMalloc((void **)&m->d_rows, sizeof(int) * (size_t)P * S)) != hipSuccess ||
+            (e = hipMalloc((void **)&m->d_rowcnt, sizeof(int) * (size_t)P)) != hipSuccess) {
+            fail(LASSIE_ERR_NOMEM, std::string("panel row lists: ") + hipGetErrorString(e));
+            return bail(LASSIE_ERR_NOMEM);
+        }
         if ((e = hipMalloc((void **)&m->d_M, sizeof(double) * (size_t)S * S)) != hipSuccess) {
             fail(LASSIE_ERR_NOMEM, std::string("dense Newton matrix: ") + hipGetErrorString(e));
             return bail(LASSIE_ERR_NOMEM);
@@ -307,9 +339,19 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     const int S = m->S;
     const int nb = lassie::dense_panel_width(S);
     const int P = (S + nb - 1) / nb;
-    int maxg = 0;
-    if (lassie::max_coop_groups(S, &maxg) != 0 || maxg < 1)
+    int cus = 0, max_lds = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->device));
+    HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device));
+    const size_t static_lds = 8 * 1024;
+    const size_t model_lds = lassie::lds_model_bytes(S, m->hptr[S]);
+    // opts.reserved bit 1 keeps the control group's model and vectors in
+    // global memory (diagnostics)
+    const bool lds_model = model_lds + static_lds <= (size_t)max_lds && !(o.reserved & 2);
+    const size_t dyn = lds_model ? model_lds : sizeof(double) * (size_t)S;
+    int per_cu = 0;
+    if (lassie::solver_occupancy(S, dyn, &per_cu) != 0 || per_cu < 1)
         return fail(LASSIE_ERR_HIP, "cannot size the cooperative launch");
+    const int G = std::max(1, std::min({P, cus * per_cu, 64}));
     lassie::SolveArgs a{};
     a.m.S = S;
     a.m.R = m->R;
@@ -321,6 +363,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.m.hk = m->d_hk;
     a.m.feed = m->d_feed;
     a.m.atol = m->d_atol;
+    a.recs = m->d_recs;
     a.x0 = m->d_x0;
     a.rkf_step = o.rkf_step;
     a.be_step = o.be_step;
@@ -338,31 +381,22 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.out = m->d_out;
     a.progress = progress_cap > 0 ? m->d_prog : nullptr;
     a.progress_cap = progress_cap;
-    double *v = m->d_vec;
-    a.y1 = v;
-    a.u = v + (size_t)S;
-    a.u2 = v + 2 * (size_t)S;
-    a.k1 = v + 3 * (size_t)S;
-    a.k2 = v + 4 * (size_t)S;
-    a.k3 = v + 5 * (size_t)S;
-    a.k4 = v + 6 * (size_t)S;
-    a.k5 = v + 7 * (size_t)S;
-    a.k6 = v + 8 * (size_t)S;
-    a.w1 = v + 9 * (size_t)S;
-    a.yv = v + 10 * (size_t)S;
-    a.x = v + 11 * (size_t)S;
+    a.gvec = m->d_vec;
     a.M = m->d_M;
     a.piv = m->d_piv;
     a.perm_src = m->d_perm;
     a.moved_to = m->d_moved;
+    a.rows = m->d_rows;
+    a.rowcnt = m->d_rowcnt;
     a.ctl = m->d_ctl;
     a.gs = m->d_gs;
     a.stats_out = m->d_stats;
     a.dstats_out = m->d_dstats;
-    a.G = std::max(1, std::min(P, maxg));
+    a.G = G;
+    a.lds_model = lds_model ? 1 : 0;
 
     HIP_TRY(hipEventRecord(m->ev0, m->stream));
-    HIP_TRY(lassie::launch_solve(a, m->stream));
+    HIP_TRY(lassie::launch_solve(a, dyn, m->stream));
     HIP_TRY(hipEventRecord(m->ev1, m->stream));
     long long st[16];
     double dst[8];
@@ -371,7 +405,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     lassie::Ctl ctl;
     HIP_TRY(hipMemcpyAsync(&ctl, m->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, m->stream));
     if (ncs) HIP_TRY(hipMemcpyAsync(out, m->d_out, sizeof(double) * (size_t)nt * ncs, hipMemcpyDeviceToHost, m->stream));
-    if (y_final) HIP_TRY(hipMemcpyAsync(y_final, a.y1, sizeof(double) * S, hipMemcpyDeviceToHost, m->stream));
+    if (y_final) HIP_TRY(hipMemcpyAsync(y_final, a.gvec, sizeof(double) * S, hipMemcpyDeviceToHost, m->stream));
     int64_t nprog = 0;
     HIP_TRY(hipStreamSynchronize(m->stream));
     if (ctl.abort) return fail(LASSIE_ERR_DEVICE, "device-side wait exceeded its bound");
@@ -400,7 +434,185 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
         stats->phase_ms[1] = dst[4];
         stats->phase_ms[2] = dst[5];
         stats->phase_ms[3] = dst[6];
+        stats->groups = G;
+        stats->lds_model = lds_model ? 1 : 0;
     }
     if (st[9]) return fail(LASSIE_ERR_MAXITER, "max_iters reached before the last sample");
     return LASSIE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: many uploaded models in one persistent launch.
+#include "batch.h"
+
+extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                                  const double *t, int32_t nt, double *out, double *y_final,
+                                  lassie_stats *stats, double *elapsed_ms)
+{
+    if (!models || n_models < 1) return fail(LASSIE_ERR_INVALID, "need at least one model");
+    if (!t || nt < 1 || !out) return fail(LASSIE_ERR_INVALID, "need sampling times and an output buffer");
+    lassie_opts o;
+    if (opts)
+        o = *opts;
+    else
+        lassie_opts_default(&o);
+    if (o.precision < LASSIE_PREC_FP64 || o.precision > LASSIE_PREC_BF16_REFINE)
+        return fail(LASSIE_ERR_INVALID, "unknown precision");
+    int smax = 1;
+    for (int i = 0; i < n_models; i++) {
+        if (!models[i]) return fail(LASSIE_ERR_INVALID, "NULL model in batch");
+        if (models[i]->S > lassie::kBatchMaxS) return fail(LASSIE_ERR_UNSUPPORTED, "batched form supports S <= 512");
+        if (models[i]->device != models[0]->device) return fail(LASSIE_ERR_INVALID, "models on different devices");
+        smax = std::max(smax, models[i]->S);
+    }
+    lassie_model *m0 = models[0];
+    HIP_TRY(hipSetDevice(m0->device));
+    std::vector<lassie::BatchInst> inst(n_models);
+    long long out_tot = 0, y_tot = 0;
+    for (int i = 0; i < n_models; i++) {
+        const lassie_model *m = models[i];
+        lassie::BatchInst &b = inst[i];
+        b.S = m->S;
+        b.R = m->R;
+        b.lptr = m->d_lptr;
+        b.lsp = m->d_lsp;
+        b.lexp = m->d_lexp;
+        b.hptr = m->d_hptr;
+        b.hrx = m->d_hrx;
+        b.recs = m->d_recs;
+        b.hk = m->d_hk;
+        b.x0 = m->d_x0;
+        b.feed = m->d_feed;
+        b.atol = m->d_atol;
+        b.out_off = out_tot;
+        b.y_off = y_tot;
+        out_tot += (long long)nt * m->S;
+        y_tot += m->S;
+    }
+    // most expensive first (dense LU work ~ S^3) so the tail of the queue is short
+    std::vector<int> order(n_models);
+    for (int i = 0; i < n_models; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].S > inst[b].S; });
+
+    int cus = 0, per_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m0->device));
+    if (lassie::batch_occupancy(&per_cu) != 0 || per_cu < 1) return fail(LASSIE_ERR_HIP, "batch occupancy query failed");
+    const int slots = std::max(1, std::min(n_models, cus * per_cu));
+    const int Pmax = (smax + lassie::kBatchNB - 1) / lassie::kBatchNB;
+    const size_t ws64_stride = 12 * (size_t)smax + (size_t)smax * smax;
+    const size_t ws32_stride = (size_t)smax * smax;
+    const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
+
+    lassie::BatchInst *d_inst = nullptr;
+    int *d_order = nullptr, *d_next = nullptr, *d_wsi = nullptr;
+    double *d_tv = nullptr, *d_out = nullptr, *d_y = nullptr, *d_ws64 = nullptr, *d_dst = nullptr;
+    float *d_ws32 = nullptr;
+    long long *d_st = nullptr;
+    int rc = LASSIE_OK;
+    auto cleanup = [&]() {
+        void *ptrs[] = {d_inst, d_order, d_next, d_wsi, d_tv, d_out, d_y, d_ws64, d_dst, d_ws32, d_st};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void **)&d_inst, sizeof(lassie::BatchInst) * n_models)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_order, sizeof(int) * n_models)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_next, sizeof(int))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_wsi, sizeof(int) * wsi_stride * slots)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_tv, sizeof(double) * nt)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_out, sizeof(double) * out_tot)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_y, sizeof(double) * y_tot)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_ws64, sizeof(double) * ws64_stride * slots)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_dst, sizeof(double) * 4 * n_models)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess ||
+        (o.precision != LASSIE_PREC_FP64 &&
+         (e = hipMalloc((void **)&d_ws32, sizeof(float) * ws32_stride * slots)) != hipSuccess)) {
+        cleanup();
+        return fail(LASSIE_ERR_NOMEM, std::string("batch workspace: ") + hipGetErrorString(e));
+    }
+    hipStream_t st = m0->stream;
+#define BATCH_TRY(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            cleanup();                                                                           \
+            return fail(LASSIE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+        }                                                                                        \
+    } while (0)
+    BATCH_TRY(hipMemcpyAsync(d_inst, inst.data(), sizeof(lassie::BatchInst) * n_models, hipMemcpyHostToDevice, st));
+    BATCH_TRY(hipMemcpyAsync(d_order, order.data(), sizeof(int) * n_models, hipMemcpyHostToDevice, st));
+    BATCH_TRY(hipMemcpyAsync(d_tv, t, sizeof(double) * nt, hipMemcpyHostToDevice, st));
+    BATCH_TRY(hipMemsetAsync(d_next, 0, sizeof(int), st));
+    BATCH_TRY(hipMemsetAsync(d_wsi, 0, sizeof(int) * wsi_stride * slots, st));
+    lassie::BatchArgs a{};
+    a.inst = d_inst;
+    a.n_inst = n_models;
+    a.order = d_order;
+    a.next = d_next;
+    a.tv = d_tv;
+    a.nt = nt;
+    a.out = d_out;
+    a.y_final = d_y;
+    a.stats = d_st;
+    a.dstats = d_dst;
+    a.rkf_step = o.rkf_step;
+    a.be_step = o.be_step;
+    a.newton_tol = o.newton_tol;
+    a.stiffness_tol = o.stiffness_tol;
+    a.rtol = o.rtol;
+    a.newton_iter = o.newton_iter;
+    a.bdf_regrow = o.bdf_regrow;
+    a.max_rkf_retries = o.max_rkf_retries;
+    a.max_iters = o.max_iters;
+    a.precision = o.precision;
+    a.refine_tol = o.refine_tol;
+    a.refine_max_iter = std::max(1, o.refine_max_iter);
+    a.s_max = smax;
+    a.ws64 = d_ws64;
+    a.ws64_stride = ws64_stride;
+    a.ws32 = d_ws32;
+    a.ws32_stride = ws32_stride;
+    a.wsi = d_wsi;
+    a.wsi_stride = wsi_stride;
+    BATCH_TRY(hipEventRecord(m0->ev0, st));
+    BATCH_TRY(lassie::launch_batch(a, slots, st));
+    BATCH_TRY(hipEventRecord(m0->ev1, st));
+    BATCH_TRY(hipMemcpyAsync(out, d_out, sizeof(double) * out_tot, hipMemcpyDeviceToHost, st));
+    if (y_final) BATCH_TRY(hipMemcpyAsync(y_final, d_y, sizeof(double) * y_tot, hipMemcpyDeviceToHost, st));
+    std::vector<long long> hst((size_t)lassie::kBatchStats * n_models);
+    std::vector<double> hd(4 * (size_t)n_models);
+    BATCH_TRY(hipMemcpyAsync(hst.data(), d_st, sizeof(long long) * hst.size(), hipMemcpyDeviceToHost, st));
+    BATCH_TRY(hipMemcpyAsync(hd.data(), d_dst, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, st));
+    BATCH_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    BATCH_TRY(hipEventElapsedTime(&ms, m0->ev0, m0->ev1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    for (int i = 0; i < n_models; i++) {
+        const long long *c = &hst[(size_t)i * lassie::kBatchStats];
+        if (stats) {
+            lassie_stats &s = stats[i];
+            std::memset(&s, 0, sizeof(s));
+            s.loop_iters = c[0];
+            s.rkf_attempts = c[1];
+            s.rkf_accepts = c[2];
+            s.bdf_steps = c[3];
+            s.newton_iters = c[4];
+            s.lu_failures = c[5];
+            s.nan_repairs = c[6];
+            s.samples = c[7];
+            s.lu_factorizations = c[8];
+            s.refine_iters = c[10];
+            s.refine_fallbacks = c[11];
+            s.t_final = hd[4 * i];
+            s.dt_final = hd[4 * i + 1];
+            s.newdt_final = hd[4 * i + 2];
+            s.kernel_ms = ms;
+            s.groups = slots;
+        }
+        if (c[9]) rc = LASSIE_ERR_MAXITER;
+    }
+    cleanup();
+    if (rc) return fail(rc, "max_iters reached before the last sample in at least one instance");
+    return LASSIE_OK;
+#undef BATCH_TRY
 }

@@ -1,0 +1,51 @@
+// Launch interface of the batched persistent solver (host <-> device).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace lassie {
+
+constexpr int kBatchMaxS = 512;  // largest instance of the batched form
+constexpr int kBatchNB = 16;     // panel width of the workgroup-local LU
+
+// One instance: device pointers of an uploaded model and where its outputs go.
+struct BatchInst {
+    int S, R;
+    const int *lptr, *lsp, *lexp, *hptr, *hrx;
+    const HRec *recs;
+    const double *hk, *x0, *feed, *atol;
+    long long out_off, y_off;
+};
+
+constexpr int kBatchStats = 12;  // counters per instance
+
+struct BatchArgs {
+    const BatchInst *inst;
+    int n_inst;
+    const int *order;  // instances, most expensive first
+    int *next;         // work-queue head
+    const double *tv;
+    int nt;
+    double *out, *y_final;
+    long long *stats;  // kBatchStats per instance
+    double *dstats;    // t, dt, newdt per instance
+    double rkf_step, be_step, newton_tol, stiffness_tol, rtol;
+    int newton_iter, bdf_regrow, max_rkf_retries;
+    long long max_iters;
+    int precision;
+    double refine_tol;
+    int refine_max_iter;
+    int s_max;
+    double *ws64;  // per slot: 12 vectors + dense M (FP64)
+    size_t ws64_stride;
+    float *ws32;   // per slot: dense M (FP32)
+    size_t ws32_stride;
+    int *wsi;      // per slot: pivots, row moves, panel row lists, flags
+    size_t wsi_stride;
+};
+
+hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream);
+int batch_occupancy(int *per_cu);
+
+}  // namespace lassie

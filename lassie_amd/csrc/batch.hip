@@ -1,0 +1,412 @@
+// Batched persistent solver: many independent models of uneven size in one
+// launch.  Every workgroup takes instances from an atomic work queue (largest
+// first) and runs LASSIE's whole integration loop (Simulation.cuh:632-925)
+// for each, with its own step control, so no workgroup ever waits for
+// another and the caches keep their contents (no grid-wide fences).
+//
+// Newton linear solve per instance, in the workgroup, row-major dense M:
+//   precision FP64   partial-pivot LU in FP64, bit-identical to the oracle
+//   FP32 / BF16      M is rounded to FP32 and factored with the same pivoting
+//                    in FP32 (BF16: trailing-update operands rounded to BF16,
+//                    FP32 accumulation); each Newton correction is refined in
+//                    FP64 against the FP64 M until ||r||_inf <= refine_tol *
+//                    ||b||_inf, else the step falls back to the FP64 LU.
+#include "batch.h"
+#include "device_common.h"
+#include "wg_common.h"
+
+namespace lassie {
+
+__device__ __forceinline__ float to_bf16_rne(float x)
+{
+    // round to the nearest BF16 value (ties to even), kept in an FP32 register
+    unsigned int u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+// Workgroup-local blocked right-looking LU of a row-major n x n matrix with
+// partial pivoting (panel_factor semantics).  Row moves go to every column,
+// so on return M holds L\U in final row order.  Returns 0 or k+1 of the
+// first zero pivot.  BF16 = round the trailing-update operands to BF16.
+template <typename T, bool BF16>
+__device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
+                        int *info, Shared &sh)
+{
+    constexpr int NBC = kBatchNB;
+    constexpr int RPT = kBatchMaxS / NT;
+    const int P = (n + NBC - 1) / NBC;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if (tid == 0) st_rlx(info, 0);
+    __syncthreads();
+    for (int p = 0; p < P; p++) {
+        const int r0 = p * NBC;
+        const int nbp = min(NBC, n - r0);
+        panel_factor<RPT, NBC, T, true>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n, rowcnt + p, info,
+                                        sh);
+        __syncthreads();
+        const int inf = ld_rlx(info);
+        if (inf) return inf;
+        for (int e = tid; e < NBC * NBC; e += NT) {
+            const int kk = e / NBC, c = e % NBC;
+            if (kk < nbp && c < kk) sh.L11[kk][c] = (double)M[(size_t)(r0 + kk) * n + r0 + c];
+        }
+        __syncthreads();
+        // row moves in every other column; U block row by forward substitution
+        int nonfinite = 0;
+        for (int c = tid; c < n; c += NT) {
+            if (c >= r0 && c < r0 + nbp) continue;
+            T vals[NBC], old[NBC];
+#pragma unroll
+            for (int kk = 0; kk < NBC; kk++) {
+                vals[kk] = (kk < nbp) ? M[(size_t)perm_src[r0 + kk] * n + c] : T(0);
+                old[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + c] : T(0);
+            }
+#pragma unroll
+            for (int kk = 0; kk < NBC; kk++) {
+                if (kk < nbp) {
+                    const int d = moved_to[r0 + kk];
+                    if (d >= r0 + nbp) M[(size_t)d * n + c] = old[kk];
+                }
+            }
+            if (c >= r0 + nbp) {
+#pragma unroll
+                for (int i = 1; i < NBC; i++) {
+                    if (i < nbp) {
+#pragma unroll
+                        for (int k = 0; k < i; k++) vals[i] = fma(-(T)sh.L11[i][k], vals[k], vals[i]);
+                    }
+                }
+#pragma unroll
+                for (int kk = 0; kk < NBC; kk++) nonfinite |= (kk < nbp) && !isfinite((double)vals[kk]);
+            }
+#pragma unroll
+            for (int kk = 0; kk < NBC; kk++)
+                if (kk < nbp) M[(size_t)(r0 + kk) * n + c] = vals[kk];
+        }
+        const bool all_rows = __syncthreads_or(nonfinite) != 0;
+        // trailing update over rows with nonzero multipliers: lanes take 64
+        // consecutive columns, waves take rows, two rows in flight per wave
+        const int c0 = r0 + nbp;
+        if (c0 < n) {
+            const int cnt = all_rows ? n - c0 : rowcnt[p];
+            const int *rl = rows + (size_t)p * n;
+            for (int cb = c0; cb < n; cb += 64) {
+                const int c = cb + lane;
+                const bool okc = c < n;
+                T u[NBC];
+#pragma unroll
+                for (int kk = 0; kk < NBC; kk++) {
+                    T uv = (okc && kk < nbp) ? M[(size_t)(r0 + kk) * n + c] : T(0);
+                    if (BF16) uv = (T)to_bf16_rne((float)uv);
+                    u[kk] = uv;
+                }
+                for (int idx = wave; idx < cnt; idx += 2 * NWAVES) {
+                    const int idx1 = idx + NWAVES;
+                    const bool h1 = idx1 < cnt;
+                    const int i0 = all_rows ? c0 + idx : rl[idx];
+                    const int i1 = h1 ? (all_rows ? c0 + idx1 : rl[idx1]) : i0;
+                    T l0[NBC], l1[NBC];
+#pragma unroll
+                    for (int kk = 0; kk < NBC; kk++) {
+                        T a = (kk < nbp) ? M[(size_t)i0 * n + r0 + kk] : T(0);
+                        T b = (kk < nbp && h1) ? M[(size_t)i1 * n + r0 + kk] : T(0);
+                        if (BF16) {
+                            a = (T)to_bf16_rne((float)a);
+                            b = (T)to_bf16_rne((float)b);
+                        }
+                        l0[kk] = a;
+                        l1[kk] = b;
+                    }
+                    T a0 = okc ? M[(size_t)i0 * n + c] : T(0);
+                    T a1 = (okc && h1) ? M[(size_t)i1 * n + c] : T(0);
+#pragma unroll
+                    for (int kk = 0; kk < NBC; kk++) {
+                        if (kk < nbp) {
+                            a0 = fma(-l0[kk], u[kk], a0);
+                            a1 = fma(-l1[kk], u[kk], a1);
+                        }
+                    }
+                    if (okc) M[(size_t)i0 * n + c] = a0;
+                    if (okc && h1) M[(size_t)i1 * n + c] = a1;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    return 0;
+}
+
+// Mixed-precision Newton correction: solve M d = rhs with the FP32 factors and
+// FP64 residuals r = rhs - M64 x.  Returns true when the refinement met the
+// tolerance; x (double, LDS) holds the result.
+__device__ bool refine_solve(const double *__restrict__ M64, const float *__restrict__ M32, int n, const int *perm_src,
+                             const int *moved_to, const double *rhs, double *x, float *b32, double tol, int max_it,
+                             long long &iters, Shared &sh)
+{
+    const int tid = threadIdx.x;
+    double bn = 0.0;
+    for (int i = tid; i < n; i += NT) {
+        x[i] = 0.0;
+        b32[i] = (float)rhs[i];
+        bn = fmax(bn, fabs(rhs[i]));
+    }
+    bn = -wg_fmin(-bn, sh);
+    for (int it = 0; it < max_it; it++) {
+        iters++;
+        lu_solve_wg<kBatchNB, float, true>(M32, n, perm_src, moved_to, b32);
+        for (int i = tid; i < n; i += NT) x[i] = x[i] + (double)b32[i];
+        __syncthreads();
+        double rn = 0.0;
+        for (int i = tid; i < n; i += NT) {
+            const double *row = M64 + (size_t)i * n;
+            double acc = rhs[i];
+            for (int j = 0; j < n; j++) acc = fma(-row[j], x[j], acc);
+            b32[i] = (float)acc;
+            rn = fmax(rn, fabs(acc));
+        }
+        rn = -wg_fmin(-rn, sh);
+        if (rn <= tol * bn) return true;
+        if (!(rn == rn)) return false;  // NaN residual: give up at once
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
+{
+    __shared__ Shared sh;
+    __shared__ double s_x[kBatchMaxS];  // solve vector / refined correction
+    __shared__ float s_b32[kBatchMaxS];
+    __shared__ int s_idx;
+    const int tid = threadIdx.x;
+    const int slot = blockIdx.x;
+    double *ws = A.ws64 + (size_t)slot * A.ws64_stride;
+    float *M32 = A.ws32 ? A.ws32 + (size_t)slot * A.ws32_stride : nullptr;
+    int *wi = A.wsi + (size_t)slot * A.wsi_stride;
+    const int smax = A.s_max;
+    const int Pmax = (smax + kBatchNB - 1) / kBatchNB;
+    int *piv = wi, *perm_src = wi + smax, *moved_to = wi + 2 * smax, *info = wi + 3 * smax;
+    int *rowcnt = wi + 3 * smax + 16, *rows = rowcnt + Pmax;
+
+    while (true) {
+        if (tid == 0) s_idx = atomicAdd(A.next, 1);
+        __syncthreads();
+        const int idx = s_idx;
+        __syncthreads();
+        if (idx >= A.n_inst) return;
+        const int inst = A.order[idx];
+        const BatchInst I = A.inst[inst];
+        const int S = I.S;
+        DevModel m;
+        m.S = S;
+        m.R = I.R;
+        m.lptr = I.lptr;
+        m.lsp = I.lsp;
+        m.lexp = I.lexp;
+        m.hptr = I.hptr;
+        m.hrx = I.hrx;
+        m.hk = I.hk;
+        m.feed = I.feed;
+        m.atol = I.atol;
+        CtlView v;
+        v.recs = I.recs;
+        v.hptr = I.hptr;
+        v.y1 = ws;
+        v.ua = ws + S;
+        v.ub = ws + 2 * S;
+        v.k1 = ws + 3 * S;
+        v.k2 = ws + 4 * S;
+        v.k3 = ws + 5 * S;
+        v.k4 = ws + 6 * S;
+        v.k5 = ws + 7 * S;
+        v.w1 = ws + 8 * S;
+        v.yv = ws + 9 * S;
+        v.x = s_x;
+        double *rhs = ws + 10 * S;
+        double *M = ws + 12 * (size_t)smax;
+        double *out = A.out + I.out_off;
+
+        double t = 0.0, t1 = A.tv[0], dt = A.rkf_step, newdt = A.be_step;
+        const double t_max = A.tv[A.nt - 1];
+        (void)t_max;
+        int i_sample = 0;
+        long long nrej = 0;
+        long long st_loop = 0, st_att = 0, st_acc = 0, st_bdf = 0, st_newton = 0, st_lufail = 0, st_rep = 0,
+                  st_lu = 0, st_ref = 0, st_fb = 0;
+        int rc = 0;
+        for (int s = tid; s < S; s += NT) v.y1[s] = I.x0[s];
+        __syncthreads();
+
+        while (true) {
+            if (A.max_iters > 0 && st_loop >= A.max_iters) {
+                rc = 1;
+                break;
+            }
+            st_loop++;
+            if (fabs(t1 - t) < 1e-5) {
+                for (int j = tid; j < S; j += NT) out[(size_t)i_sample * S + j] = v.y1[j];
+                i_sample++;
+                if (i_sample == A.nt) break;
+                t1 = A.tv[i_sample];
+            }
+            st_att++;
+            double mind;
+            const int nok = rkf_attempt_wg(m, A.rtol, v, dt, &mind, sh);
+            if (nok == S) {
+                nrej = 0;
+                st_acc++;
+                for (int s = tid; s < S; s += NT) v.y1[s] = v.w1[s];
+                __syncthreads();
+                const double d_temp = dt * mind;
+                if ((t + d_temp) > t1)
+                    dt = fabs(t1 - t);
+                else
+                    dt = d_temp;
+                t = t + dt;
+                continue;
+            }
+            const double dt_old = dt;
+            dt = dt * mind;
+            nrej++;
+            if (!(dt < A.stiffness_tol || isnan(dt) || isinf(dt) || fabs(dt_old - dt) < 1e-3 ||
+                  (A.max_rkf_retries > 0 && nrej >= A.max_rkf_retries)))
+                continue;
+            // ---- BDF1 step with modified Newton (Simulation.cuh:780-919) ------
+            nrej = 0;
+            st_bdf++;
+            dt = newdt;
+            if ((t + dt) > t1) dt = fabs(t1 - t);
+            const double h = dt;
+            for (int s = tid; s < S; s += NT) v.yv[s] = v.y1[s];
+            __syncthreads();
+            bool luFail = false;
+            if (A.newton_iter > 0) {
+                const size_t nn = (size_t)S * S;
+                for (size_t e = tid; e < nn; e += NT) M[e] = 0.0;
+                __syncthreads();
+                for (int r = tid; r < S; r += NT) jac_row<true>(m, v.yv, M, S, r, h);
+                __syncthreads();
+                bool fp64 = A.precision == 0;
+                int inf = 0;
+                if (!fp64) {
+                    for (size_t e = tid; e < nn; e += NT) M32[e] = (float)M[e];
+                    __syncthreads();
+                    const int inf32 = (A.precision == 2)
+                                          ? wg_lu_rm<float, true>(M32, S, piv, perm_src, moved_to, rows, rowcnt, info, sh)
+                                          : wg_lu_rm<float, false>(M32, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                    if (inf32) {
+                        fp64 = true;
+                        st_fb++;
+                    }
+                }
+                if (fp64) inf = wg_lu_rm<double, false>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                st_lu++;
+                int itN = 0;
+                double errN = 1000;
+                while (itN < A.newton_iter && errN > A.newton_tol) {
+                    for (int s = tid; s < S; s += NT) {
+                        const double f = rhs_species(m, v.recs, v.hptr, v.yv, s);
+                        rhs[s] = -(v.yv[s] - v.y1[s] - h * f);
+                    }
+                    __syncthreads();
+                    if (inf) {
+                        for (int s = tid; s < S; s += NT)
+                            if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
+                        __syncthreads();
+                        newdt = h * 0.1;
+                        luFail = true;
+                        st_lufail++;
+                        break;
+                    }
+                    if (fp64) {
+                        for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
+                        __syncthreads();
+                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x);
+                    } else if (!refine_solve(M, M32, S, perm_src, moved_to, rhs, s_x, s_b32, A.refine_tol,
+                                             A.refine_max_iter, st_ref, sh)) {
+                        // refinement did not converge: FP64 LU for the rest of this step
+                        fp64 = true;
+                        st_fb++;
+                        inf = wg_lu_rm<double, false>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                        if (inf) continue;  // handled at the top of the loop as an LU failure
+                        for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
+                        __syncthreads();
+                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x);
+                    }
+                    double part = 0.0;
+                    for (int s = tid; s < S; s += NT) {
+                        const double yv1 = v.yv[s];
+                        double yv;
+                        if (m.feed[s] == 0)
+                            yv = yv1 + s_x[s];
+                        else
+                            yv = m.feed[s];
+                        v.yv[s] = yv;
+                        const double d = yv - yv1;
+                        part = part + (double)(float)(d * d);
+                    }
+                    errN = sqrt(wg_tree_sum(part, sh));
+                    itN = itN + 1;
+                }
+                st_newton += itN;
+            }
+            if (!luFail) {
+                int nbad = 0;
+                for (int s = tid; s < S; s += NT) nbad += bad_value(v.yv[s]) ? 1 : 0;
+                nbad = wg_isum(nbad, sh);
+                if (nbad > 0) {
+                    for (int s = tid; s < S; s += NT)
+                        if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
+                    newdt = h * 0.1;
+                    st_rep++;
+                } else if (A.bdf_regrow) {
+                    newdt = fmin(newdt * 2.0, A.be_step);
+                }
+            }
+            __syncthreads();
+            for (int s = tid; s < S; s += NT) v.y1[s] = v.yv[s];
+            __syncthreads();
+            t = t + h;
+        }
+        for (int s = tid; s < S; s += NT) A.y_final[I.y_off + s] = v.y1[s];
+        if (tid == 0) {
+            long long *o = A.stats + (size_t)inst * kBatchStats;
+            o[0] = st_loop;
+            o[1] = st_att;
+            o[2] = st_acc;
+            o[3] = st_bdf;
+            o[4] = st_newton;
+            o[5] = st_lufail;
+            o[6] = st_rep;
+            o[7] = i_sample;
+            o[8] = st_lu;
+            o[9] = rc;
+            o[10] = st_ref;
+            o[11] = st_fb;
+            double *d = A.dstats + (size_t)inst * 4;
+            d[0] = t;
+            d[1] = dt;
+            d[2] = newdt;
+        }
+        __syncthreads();
+    }
+}
+
+int batch_occupancy(int *per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, reinterpret_cast<const void *>(&batch_kernel), NT,
+                                                        0) == hipSuccess
+               ? 0
+               : -1;
+}
+
+hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream)
+{
+    hipLaunchKernelGGL(batch_kernel, dim3(slots), dim3(NT), 0, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace lassie

@@ -32,54 +32,81 @@ struct DevModel {
     const double *feed, *atol;
 };
 
-// f_s of generate_ode (kernels.cuh:5-50) for one species, same operation order.
-__device__ __forceinline__ double rhs_species(const DevModel &m, const double *__restrict__ x, int s)
+// One net-stoichiometry entry with its reaction's reactants inlined, so a
+// right-hand-side term needs one record read instead of a chain of dependent
+// index loads.  kind 0: no reactant; 1, 2: that many reactants, species in
+// sp01 (16 bits each), exponents in meta bits 0-7 and 8-15; 3: any other
+// reaction, sp01 holds the reaction index for the CSR walk.
+struct HRec {
+    double hk;
+    int sp01;
+    int meta;
+};
+
+__device__ __forceinline__ double rhs_species_generic(const DevModel &m, const double *x, int r, double sk)
+{
+    const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
+    if (a1 <= a0) return sk;
+    double temp = 0.0;
+    for (int j = a0; j < a1; j++) {
+        const double p = mypow(x[m.lsp[j]], m.lexp[j]);
+        if (j == a0)
+            temp = temp + sk * p;
+        else
+            temp = temp * p;
+    }
+    return temp;
+}
+
+// f_s of generate_ode (kernels.cuh:5-50) for one species, same operation
+// order: acc = ((0 + t_0) + t_1) + ..., t = ((0 + sk*x_a^ea) * x_b^eb).
+__device__ __forceinline__ double rhs_species(const DevModel &m, const HRec *recs, const int *hptr,
+                                              const double *x, int s)
 {
     double acc = 0.0;
-    const int e1 = m.hptr[s + 1];
-    for (int e = m.hptr[s]; e < e1; e++) {
-        const int r = m.hrx[e];
-        const double sk = m.hk[e];
-        const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
-        if (a1 > a0) {
+    const int e1 = hptr[s + 1];
+    for (int e = hptr[s]; e < e1; e++) {
+        const HRec rc = recs[e];
+        const int kind = rc.meta >> 16;
+        if (kind == 0) {
+            acc = acc + rc.hk;
+        } else if (kind <= 2) {
             double temp = 0.0;
-            for (int j = a0; j < a1; j++) {
-                const double p = mypow(x[m.lsp[j]], m.lexp[j]);
-                if (j == a0)
-                    temp = temp + sk * p;
-                else
-                    temp = temp * p;
-            }
+            temp = temp + rc.hk * mypow(x[rc.sp01 & 0xffff], rc.meta & 0xff);
+            if (kind == 2) temp = temp * mypow(x[(unsigned)rc.sp01 >> 16], (rc.meta >> 8) & 0xff);
             acc = acc + temp;
         } else {
-            acc = acc + sk;
+            acc = acc + rhs_species_generic(m, x, rc.sp01, rc.hk);
         }
     }
     return acc;
 }
 
-// ---- scoped atomics for control words shared between workgroups ----------
-__device__ __forceinline__ int ld_acq(const int *p)
+// ---- control words shared between workgroups (relaxed, agent scope) -------
+// Ordering of everything else is provided by grid_barrier.
+__device__ __forceinline__ int ld_rlx(const int *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_rel(int *p, int v)
+__device__ __forceinline__ void st_rlx(int *p, int v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ double ld_acq_d(const double *p)
+__device__ __forceinline__ double ld_rlx_d(const double *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_rel_d(double *p, double v)
+__device__ __forceinline__ void st_rlx_d(double *p, double v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Grid-wide barrier for a cooperative launch (every workgroup resident).
-// Sense by generation counter; each wait is bounded: when the bound is hit the
-// abort word is raised and the caller leaves the kernel, so no wave can spin
-// forever.  Returns false on abort.
+// Release and acquire are agent-scope fences (L2 write-back / invalidate), the
+// only scope that orders plain global stores of one CU before plain loads on
+// another.  Each wait is bounded: when the bound is hit the abort word is
+// raised and the caller leaves the kernel, so no wave can spin forever.
+// Returns false on abort.
 struct GridSync {
     unsigned int count;
     unsigned int pad0[31];
@@ -87,7 +114,8 @@ struct GridSync {
     unsigned int pad1[31];
 };
 
-constexpr long long kSpinLimit = 1LL << 27;  // ~ several seconds of s_sleep(1)
+// bound on one wait, in ticks of the 100 MHz wall clock (60 s)
+constexpr unsigned long long kWaitLimit = 6000000000ULL;
 
 __device__ __forceinline__ bool grid_barrier(GridSync *gs, unsigned int nwg, unsigned int &gen_local,
                                              int *abort_word)
@@ -99,28 +127,32 @@ __device__ __forceinline__ bool grid_barrier(GridSync *gs, unsigned int nwg, uns
         const unsigned int my = gen_local;
         __threadfence();
         const unsigned int arrived =
-            __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nwg - 1) {
             __hip_atomic_store(&gs->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gs->gen, my + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            __hip_atomic_store(&gs->gen, my + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            long long spins = 0;
-            while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    st_rel(abort_word, 1);
-                    ok = 0;
-                    break;
-                }
-                if ((spins & 1023) == 0 && ld_acq(abort_word)) {
-                    ok = 0;
-                    break;
+            const unsigned long long t0 = wall_clock64();
+            unsigned int spins = 0;
+            while (__hip_atomic_load(&gs->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == my) {
+                __builtin_amdgcn_s_sleep(2);
+                if ((++spins & 255) == 0) {
+                    if (wall_clock64() - t0 > kWaitLimit) {
+                        st_rlx(abort_word, 1);
+                        ok = 0;
+                        break;
+                    }
+                    if (ld_rlx(abort_word)) {
+                        ok = 0;
+                        break;
+                    }
                 }
             }
         }
         __threadfence();
         gen_local = my + 1;
-        if (ok && ld_acq(abort_word)) ok = 0;
+        if (ok && ld_rlx(abort_word)) ok = 0;
         s_ok = ok;
     }
     __syncthreads();

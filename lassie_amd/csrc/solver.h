@@ -7,23 +7,27 @@
 namespace lassie {
 
 constexpr int kMaxDenseN = 4096;  // largest S for the dense Newton matrix
+constexpr int kNumVecLds = 11;    // state vectors the control group keeps in LDS
 
 enum { MODE_CONTINUE = 0, MODE_BDF = 1, MODE_BDF_NOOP = 2, MODE_DONE = 3 };
 
-// Control words exchanged between workgroups (one cache line each).
+// Control words exchanged between workgroups (one cache line each).  mode
+// and h are double-buffered by loop-iteration parity: group 0 may publish
+// iteration i+1 while a slower group still reads iteration i.
 struct Ctl {
-    int mode;
-    int pad0[15];
+    int mode[2];
+    int pad0[14];
     int info;
     int pad1[15];
     int abort;
     int pad2[15];
-    double h;
-    double pad3[7];
+    double h[2];
+    double pad4[6];
 };
 
 struct SolveArgs {
     DevModel m;
+    const HRec *recs;
     const double *x0;
     double rkf_step, be_step, newton_tol, stiffness_tol, rtol;
     int newton_iter, bdf_regrow, max_rkf_retries;
@@ -35,18 +39,22 @@ struct SolveArgs {
     double *out;
     double *progress;
     long long progress_cap;
-    double *y1, *u, *u2, *k1, *k2, *k3, *k4, *k5, *k6, *w1, *yv, *x;
+    // state vectors in global memory (S each): y1 u u2 k1..k5 w1 yv x
+    double *gvec;
     double *M;
     int *piv, *perm_src, *moved_to;
+    int *rows, *rowcnt;    // per panel: positions with nonzero multipliers
     Ctl *ctl;
     GridSync *gs;
     long long *stats_out;  // 10 counters
-    double *dstats_out;    // t, dt, newdt
-    int G;
+    double *dstats_out;    // t, dt, newdt, phase ms x4
+    int G;                 // workgroups in the cooperative launch
+    int lds_model;         // 1: control group keeps model + vectors in LDS
 };
 
-hipError_t launch_solve(const SolveArgs &args, hipStream_t stream);
-int max_coop_groups(int n, int *out);
+hipError_t launch_solve(const SolveArgs &args, size_t dyn_lds, hipStream_t stream);
+int solver_occupancy(int n, size_t dyn_lds, int *per_cu);
 int dense_panel_width(int n);
+size_t lds_model_bytes(int S, int nnz_h);
 
 }  // namespace lassie

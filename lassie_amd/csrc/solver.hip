@@ -1,377 +1,57 @@
 // Persistent single-model solver: the whole LASSIE integration loop
 // (Simulation.cuh:632-925) in one cooperative launch.
 //
-//   workgroup 0   sample check, RKF45 attempt, step control, Newton loop
-//   all groups    dense Jacobian assembly of M = I - hJ and the blocked LU
+//   group 0      sample check, RKF45 attempt, step control, Newton loop; keeps
+//                the model and the state vectors in LDS when they fit
+//   all groups   dense assembly of M = I - hJ and the blocked LU
 //
 // Every floating-point operation follows the order written in the reference
 // kernels (kernels.cuh) with no contraction (built with -ffp-contract=off),
 // so the trajectory is bit-identical to the CPU restatement used as oracle.
 #include "device_common.h"
 #include "solver.h"
+#include "wg_common.h"
 
 namespace lassie {
 
-struct Shared {
-    double red[NT];
-    int ired[NT];
-    double cand_row[2][NWAVES][16];
-    double cand_v[2][NWAVES];
-    int cand_p[2][NWAVES];
-    double U12[16][16];
-    double L11[16][16];
-    double oldP[16][16];
-    int iscal[4];
-    double b[kMaxDenseN];
-};
-
-// ---------------------------------------------------------------------------
-// workgroup reductions (all threads receive the result)
-
-// sum over lanes in the fixed "tid strided, then halving tree" order used for
-// the Newton norm (Simulation.cuh:883-888 with a float-returning square).
-__device__ double wg_tree_sum(double v, Shared &sh)
+// Row moves of panel p applied to a strip left of it (q < p), so that every
+// L column ends in the final row order (LAPACK's dlaswp on the left part).
+template <int NBC>
+__device__ void strip_moves(double *__restrict__ M, int n, int p, int q, const int *perm_src,
+                            const int *moved_to, Shared &sh)
 {
     const int tid = threadIdx.x;
-    sh.red[tid] = v;
-    __syncthreads();
-    for (int stride = NT / 2; stride > 0; stride >>= 1) {
-        if (tid < stride) sh.red[tid] = sh.red[tid] + sh.red[tid + stride];
-        __syncthreads();
-    }
-    const double r = sh.red[0];
-    __syncthreads();
-    return r;
-}
-
-__device__ double wg_fmin(double v, Shared &sh)
-{
-    const int tid = threadIdx.x;
-    sh.red[tid] = v;
-    __syncthreads();
-    for (int stride = NT / 2; stride > 0; stride >>= 1) {
-        if (tid < stride) sh.red[tid] = fmin(sh.red[tid], sh.red[tid + stride]);
-        __syncthreads();
-    }
-    const double r = sh.red[0];
-    __syncthreads();
-    return r;
-}
-
-__device__ int wg_isum(int v, Shared &sh)
-{
-    const int tid = threadIdx.x;
-    sh.ired[tid] = v;
-    __syncthreads();
-    for (int stride = NT / 2; stride > 0; stride >>= 1) {
-        if (tid < stride) sh.ired[tid] = sh.ired[tid] + sh.ired[tid + stride];
-        __syncthreads();
-    }
-    const int r = sh.ired[0];
-    __syncthreads();
-    return r;
-}
-
-// ---------------------------------------------------------------------------
-// RKF45 attempt (Simulation.cuh:692-740; kernels.cuh:68-199), workgroup-wide.
-// Returns the number of species passing the test; *mind = min of the step
-// factors with thrust::min_element semantics (a NaN first element wins,
-// other NaNs are ignored).
-__device__ int rkf_attempt_wg(const SolveArgs &A, double dt, double *mind, Shared &sh)
-{
-    const DevModel &m = A.m;
-    const int S = m.S;
-    const int tid = threadIdx.x;
-    double *__restrict__ y1 = A.y1;
-    double *__restrict__ ua = A.u;
-    double *__restrict__ ub = A.u2;
-    double *k1 = A.k1, *k2 = A.k2, *k3 = A.k3, *k4 = A.k4, *k5 = A.k5;
-    const double *fd = m.feed;
-
-    for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, y1, s);
-        if (fd[s] != 0.0) {
-            k1[s] = fd[s];
-            ua[s] = fd[s];
-        } else {
-            const double kk = f * dt;
-            k1[s] = kk;
-            ua[s] = y1[s] + kk / 4.0;
-        }
-    }
-    __syncthreads();
-    for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, ua, s);
-        if (fd[s] != 0.0) {
-            k2[s] = fd[s];
-            ub[s] = fd[s];
-        } else {
-            const double kk = f * dt;
-            k2[s] = kk;
-            ub[s] = y1[s] + (3.0 * k1[s]) / 32.0 + (9.0 * kk) / 32.0;
-        }
-    }
-    __syncthreads();
-    for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, ub, s);
-        if (fd[s] != 0.0) {
-            k3[s] = fd[s];
-            ua[s] = fd[s];
-        } else {
-            const double kk = f * dt;
-            k3[s] = kk;
-            ua[s] = y1[s] + (1932.0 * k1[s]) / 2197.0 - (7200.0 * k2[s]) / 2197.0 + (7296.0 * kk) / 2197.0;
-        }
-    }
-    __syncthreads();
-    for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, ua, s);
-        if (fd[s] != 0.0) {
-            k4[s] = fd[s];
-            ub[s] = fd[s];
-        } else {
-            const double kk = f * dt;
-            k4[s] = kk;
-            ub[s] = y1[s] + (439.0 * k1[s]) / 216.0 - (8.0 * k2[s]) + (3680.0 * k3[s]) / 513.0
-                    - (845.0 * kk) / 4104.0;
-        }
-    }
-    __syncthreads();
-    for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, ub, s);
-        if (fd[s] != 0.0) {
-            k5[s] = fd[s];
-            ua[s] = fd[s];
-        } else {
-            const double kk = f * dt;
-            k5[s] = kk;
-            ua[s] = y1[s] - (8.0 * k1[s]) / 27.0 + (2.0 * k2[s]) - (3544.0 * k3[s]) / 2565.0
-                    + (1859.0 * k4[s]) / 4104.0 - (11.0 * kk) / 40.0;
-        }
-    }
-    __syncthreads();
-    int nok = 0;
-    double dmin = INFINITY;
-    for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, ua, s);
-        double w1, w2;
-        if (fd[s] != 0.0) {
-            w1 = fd[s];
-            w2 = fd[s];
-        } else {
-            const double kk6 = f * dt;
-            w1 = y1[s] + (25.0 * k1[s]) / 216.0 + (1408.0 * k3[s]) / 2565.0 + (2197.0 * k4[s]) / 4104.0
-                 - k5[s] / 5.0;
-            w2 = y1[s] + (16.0 * k1[s]) / 135.0 + (6656.0 * k3[s]) / 12825.0
-                 + (28561.0 * k4[s]) / 56430.0 - (9.0 * k5[s] / 50.0) + (2.0 * kk6) / 55.0;
-        }
-        A.w1[s] = w1;
-        const double R = fabs(w1 - w2) / dt;
-        const double tol = (A.rtol > 0.0) ? m.atol[s] + A.rtol * fabs(w1) : m.atol[s];
-        const double d = 0.84 * sqrt(sqrt(tol / R));
-        if (s == 0) sh.iscal[0] = isnan(d) ? 1 : 0;
-        dmin = fmin(dmin, d);
-        if (R <= tol && !isnan(R) && !isinf(R) && w1 > 0) nok++;
-    }
-    nok = wg_isum(nok, sh);
-    double r = wg_fmin(dmin, sh);
-    if (sh.iscal[0]) r = NAN;
-    *mind = r;
-    return nok;
-}
-
-// ---------------------------------------------------------------------------
-// Jacobian (createJac_, kernels.cuh:219-271) and M = I - hJ
-// (aggiorna_Jacobiano1, kernels.cuh:393-402) for row i, into the column-major
-// dense M that was zero-filled beforehand.  Rows are owned by one thread, so
-// the accumulation order per entry is the reference's (reaction ascending).
-__device__ void jac_row(const DevModel &m, const double *__restrict__ x, double *__restrict__ M, int n,
-                        int i, double h)
-{
-    const int e0 = m.hptr[i], e1 = m.hptr[i + 1];
-    for (int e = e0; e < e1; e++) {
-        const int r = m.hrx[e];
-        const double sk = m.hk[e];
-        const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
-        for (int j = a0; j < a1; j++) {
-            const int g = m.lsp[j];
-            const int ex = m.lexp[j];
-            double temp = 0.0;
-            if (ex == 1)
-                temp = temp + sk;
-            else if (ex > 1)
-                temp = temp + (sk * (double)ex) * mypow(x[g], ex - 1);
-            else
-                continue;
-            for (int jj = a0; jj < a1; jj++)
-                if (m.lsp[jj] != g) temp = temp * x[m.lsp[jj]];
-            double *p = M + (size_t)g * n + i;
-            *p = *p + temp;
-        }
-    }
-    double *d = M + (size_t)i * n + i;
-    *d = 1.0 - h * *d;
-    for (int e = e0; e < e1; e++) {
-        const int r = m.hrx[e];
-        const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
-        for (int j = a0; j < a1; j++) {
-            const int g = m.lsp[j];
-            if (m.lexp[j] < 1 || g == i) continue;
-            bool dup = false;
-            for (int e2 = e0; e2 <= e && !dup; e2++) {
-                const int r2 = m.hrx[e2];
-                const int b0 = m.lptr[r2], b1 = m.lptr[r2 + 1];
-                for (int j2 = b0; j2 < b1; j2++) {
-                    if (e2 == e && j2 >= j) break;
-                    if (m.lsp[j2] == g && m.lexp[j2] >= 1) {
-                        dup = true;
-                        break;
-                    }
-                }
-            }
-            if (dup) continue;
-            double *p = M + (size_t)g * n + i;
-            *p = 0.0 - h * *p;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Panel factorization with partial pivoting of columns [r0, r0+nbp), rows
-// [r0, n), held in registers: RPT rows per thread, NBC columns.  Pivot: the
-// largest |a| among remaining rows, first (lowest position) on ties, a NaN at
-// the diagonal position wins (LAPACK idamax / dgetf2 semantics).  Row swaps
-// are tracked as positions and applied when the panel is written back.
-// Publishes piv[k], perm_src[pos] (original row of the pivot now at pos) and
-// moved_to[r0+j] (final position of the row that started at r0+j).
-template <int RPT, int NBC>
-__device__ void panel_factor(double *__restrict__ M, int n, int p, int *piv, int *perm_src, int *moved_to,
-                             int *info, Shared &sh)
-{
     const int r0 = p * NBC;
     const int nbp = min(NBC, n - r0);
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    double a[RPT][NBC];
-    int pos[RPT];
-    bool act[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int r = r0 + tid + NT * q;
-        pos[q] = r;
-        act[q] = r < n;
-#pragma unroll
-        for (int c = 0; c < NBC; c++) a[q][c] = (r < n && c < nbp) ? M[(size_t)(r0 + c) * n + r] : 0.0;
-    }
-    int buf = 0;
-    bool stop = false;
-#pragma unroll
-    for (int j = 0; j < NBC; j++) {
-        if (j < nbp && !stop) {
-        const int k = r0 + j;
-        double bv = -2.0;
-        int bp = 0x7fffffff;
-        int bq = -1;
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            if (act[q]) {
-                const double xv = a[q][j];
-                double v = fabs(xv);
-                if (isnan(xv)) v = (pos[q] == k) ? INFINITY : -1.0;
-                if (v > bv || (v == bv && pos[q] < bp)) {
-                    bv = v;
-                    bp = pos[q];
-                    bq = q;
-                }
-            }
-        }
-        const int mybp = bp;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(bv, off);
-            const int op = __shfl_xor(bp, off);
-            if (ov > bv || (ov == bv && op < bp)) {
-                bv = ov;
-                bp = op;
-            }
-        }
-        if (lane == 0) {
-            sh.cand_v[buf][wave] = bv;
-            sh.cand_p[buf][wave] = bp;
-        }
-        if (bq >= 0 && mybp == bp) {
-#pragma unroll
-            for (int q = 0; q < RPT; q++)
-                if (q == bq) {
-#pragma unroll
-                    for (int c = 0; c < NBC; c++) sh.cand_row[buf][wave][c] = a[q][c];
-                }
-        }
-        __syncthreads();
-        int w = 0;
-        bv = sh.cand_v[buf][0];
-        bp = sh.cand_p[buf][0];
-#pragma unroll
-        for (int ww = 1; ww < NWAVES; ww++) {
-            const double ov = sh.cand_v[buf][ww];
-            const int op = sh.cand_p[buf][ww];
-            if (ov > bv || (ov == bv && op < bp)) {
-                bv = ov;
-                bp = op;
-                w = ww;
-            }
-        }
-        const double *prow = sh.cand_row[buf][w];
-        const double ukk = prow[j];
-        if (ukk == 0.0) {
-            if (tid == 0) st_rel(info, k + 1);
-            stop = true;
-        } else {
-        if (tid == 0) piv[k] = bp;
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            const int old = pos[q];
-            if (act[q] && old == bp) {
-                act[q] = false;
-                pos[q] = k;
-            } else if (act[q] && old == k) {
-                pos[q] = bp;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            if (act[q]) {
-                const double l = a[q][j] / ukk;
-                a[q][j] = l;
-#pragma unroll
-                for (int c = 0; c < NBC; c++)
-                    if (c > j) a[q][c] = fma(-l, prow[c], a[q][c]);
-            }
-        }
-        buf ^= 1;
-        }
+    const int c0 = q * NBC;
+    const int ncols = min(NBC, n - c0);
+    for (int e = tid; e < NBC * NBC; e += NT) {
+        const int kk = e / NBC, c = e % NBC;
+        if (kk < nbp && c < ncols) {
+            sh.U12[kk][c] = M[(size_t)(c0 + c) * n + perm_src[r0 + kk]];
+            sh.oldP[kk][c] = M[(size_t)(c0 + c) * n + r0 + kk];
         }
     }
-    if (stop) return;
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int orig = r0 + tid + NT * q;
-        if (orig < n) {
-#pragma unroll
-            for (int c = 0; c < NBC; c++)
-                if (c < nbp) M[(size_t)(r0 + c) * n + pos[q]] = a[q][c];
-            if (!act[q]) perm_src[pos[q]] = orig;
-            if (orig < r0 + nbp) moved_to[orig] = pos[q];
+    __syncthreads();
+    for (int e = tid; e < NBC * NBC; e += NT) {
+        const int kk = e / NBC, c = e % NBC;
+        if (kk < nbp && c < ncols) {
+            M[(size_t)(c0 + c) * n + r0 + kk] = sh.U12[kk][c];
+            const int dst = moved_to[r0 + kk];
+            if (dst >= r0 + nbp) M[(size_t)(c0 + c) * n + dst] = sh.oldP[kk][c];
         }
     }
+    __syncthreads();
 }
 
-// Apply panel p (swaps, triangular solve for the U block row, trailing update)
-// to strip q (columns [q*NBC, q*NBC + ncols)), q > p.  Owned by one workgroup.
+// Apply panel p (row moves, triangular solve for the U block row, trailing
+// update) to strip q (columns [q*NBC, q*NBC + ncols)), q > p.  One group.
+// The trailing update visits only the rows listed by the panel (all others
+// have zero multipliers, so a_ij = fma(-0, u, a_ij) = a_ij), unless the U
+// block holds a non-finite value, in which case every row is visited.
 template <int NBC>
 __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const int *perm_src,
-                             const int *moved_to, Shared &sh)
+                             const int *moved_to, const int *rows, int rowcnt, Shared &sh)
 {
     const int tid = threadIdx.x;
     const int r0 = p * NBC;
@@ -406,13 +86,18 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
         }
     }
     __syncthreads();
+    int nonfinite = 0;
     for (int e = tid; e < NBC * NBC; e += NT) {
         const int kk = e / NBC, c = e % NBC;
-        if (kk < nbp && c < ncols) M[(size_t)(c0 + c) * n + r0 + kk] = sh.U12[kk][c];
+        if (kk < nbp && c < ncols) {
+            M[(size_t)(c0 + c) * n + r0 + kk] = sh.U12[kk][c];
+            nonfinite |= !isfinite(sh.U12[kk][c]);
+        }
     }
+    const bool all_rows = __syncthreads_or(nonfinite) != 0;
     // trailing update A22 -= L21 * U12, fma chain with k ascending.  A wave
-    // walks 64 consecutive rows (coalesced in the column-major layout) and
-    // owns CPW columns of the strip, whose U12 entries stay in registers.
+    // takes 64 rows at a time and owns CPW columns of the strip, whose U12
+    // entries stay in registers; two row blocks are in flight per iteration.
     constexpr int CPW = NBC / NWAVES;
     const int lane = tid & 63;
     const int cbase = (tid >> 6) * CPW;
@@ -421,146 +106,100 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
     for (int kk = 0; kk < NBC; kk++)
 #pragma unroll
         for (int cc = 0; cc < CPW; cc++) u[kk][cc] = (kk < nbp) ? sh.U12[kk][cbase + cc] : 0.0;
-    for (int i = r0 + nbp + lane; i < n; i += 64) {
-        double acc[CPW];
+    const int nr = all_rows ? n - (r0 + nbp) : rowcnt;
+    for (int x0 = lane; x0 < nr; x0 += 128) {
+        const int x1 = x0 + 64;
+        const bool has1 = x1 < nr;
+        const int i0 = all_rows ? r0 + nbp + x0 : rows[x0];
+        const int i1 = has1 ? (all_rows ? r0 + nbp + x1 : rows[x1]) : i0;
+        double acc0[CPW], acc1[CPW], l0[NBC], l1[NBC];
 #pragma unroll
-        for (int cc = 0; cc < CPW; cc++)
-            acc[cc] = (cbase + cc < ncols) ? M[(size_t)(c0 + cbase + cc) * n + i] : 0.0;
+        for (int cc = 0; cc < CPW; cc++) {
+            const bool okc = cbase + cc < ncols;
+            acc0[cc] = okc ? M[(size_t)(c0 + cbase + cc) * n + i0] : 0.0;
+            acc1[cc] = (okc && has1) ? M[(size_t)(c0 + cbase + cc) * n + i1] : 0.0;
+        }
+#pragma unroll
+        for (int kk = 0; kk < NBC; kk++) {
+            l0[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + i0] : 0.0;
+            l1[kk] = (kk < nbp && has1) ? M[(size_t)(r0 + kk) * n + i1] : 0.0;
+        }
 #pragma unroll
         for (int kk = 0; kk < NBC; kk++) {
             if (kk < nbp) {
-                const double l = M[(size_t)(r0 + kk) * n + i];
 #pragma unroll
-                for (int cc = 0; cc < CPW; cc++) acc[cc] = fma(-l, u[kk][cc], acc[cc]);
+                for (int cc = 0; cc < CPW; cc++) {
+                    acc0[cc] = fma(-l0[kk], u[kk][cc], acc0[cc]);
+                    acc1[cc] = fma(-l1[kk], u[kk][cc], acc1[cc]);
+                }
             }
         }
 #pragma unroll
-        for (int cc = 0; cc < CPW; cc++)
-            if (cbase + cc < ncols) M[(size_t)(c0 + cbase + cc) * n + i] = acc[cc];
+        for (int cc = 0; cc < CPW; cc++) {
+            if (cbase + cc < ncols) {
+                M[(size_t)(c0 + cbase + cc) * n + i0] = acc0[cc];
+                if (has1) M[(size_t)(c0 + cbase + cc) * n + i1] = acc1[cc];
+            }
+        }
     }
     __syncthreads();
 }
 
-// Blocked right-looking LU over all workgroups; strips are owned round-robin.
+// Blocked right-looking LU over all groups; strips are owned round-robin.
 // The owner of strip p+1 updates it first and factors panel p+1 at once
-// (look-ahead), so each panel costs one grid barrier.  Returns false on abort.
+// (look-ahead), so each panel costs one grid barrier; left strips receive the
+// panel's row moves.  On return M holds L\U in final row order.  Returns
+// false on abort.
 template <int RPT, int NBC>
-__device__ bool lu_factor_grid(const SolveArgs &A, Shared &sh, unsigned int &gen_local, bool &singular)
+__device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned int &gen_local,
+                               bool &singular)
 {
     const int n = A.m.S;
     const int P = (n + NBC - 1) / NBC;
     const int G = A.G;
-    const int wg = blockIdx.x;
     singular = false;
     // A zero pivot in panel p+1 may be recorded while slower groups are still
     // checking the flag for panel p, so each check only counts columns of
     // panels that were complete at the preceding barrier.
     auto failed_before = [&](int col_end) {
-        const int inf = ld_acq(&A.ctl->info);
+        const int inf = ld_rlx(&A.ctl->info);
         return inf != 0 && inf - 1 < col_end;
     };
-    if (wg == 0) panel_factor<RPT, NBC>(A.M, n, 0, A.piv, A.perm_src, A.moved_to, &A.ctl->info, sh);
+    if (wg == 0)
+        panel_factor<RPT, NBC, double, false>(A.M, n, 0, A.piv, A.perm_src, A.moved_to, A.rows, A.rowcnt, &A.ctl->info, sh);
     if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
     if (failed_before(NBC)) {
         singular = true;
         return true;
     }
     for (int p = 0; p < P; p++) {
+        const int cnt = A.rowcnt[p];
+        const int *rows = A.rows + (size_t)p * n;
         int q = p + 1 + ((wg - (p + 1)) % G + G) % G;
         for (; q < P; q += G) {
-            strip_update<NBC>(A.M, n, p, q, A.perm_src, A.moved_to, sh);
+            strip_update<NBC>(A.M, n, p, q, A.perm_src, A.moved_to, rows, cnt, sh);
             if (q == p + 1) {
                 __syncthreads();
-                panel_factor<RPT, NBC>(A.M, n, p + 1, A.piv, A.perm_src, A.moved_to, &A.ctl->info, sh);
+                panel_factor<RPT, NBC, double, false>(A.M, n, p + 1, A.piv, A.perm_src, A.moved_to,
+                                       A.rows + (size_t)(p + 1) * n, A.rowcnt + p + 1, &A.ctl->info, sh);
                 __syncthreads();
             }
         }
-        if (p + 1 < P) {
-            if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
-            if (failed_before((p + 2) * NBC)) {
-                singular = true;
-                return true;
-            }
+        for (int ql = wg; ql < p; ql += G) strip_moves<NBC>(A.M, n, p, ql, A.perm_src, A.moved_to, sh);
+        if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
+        if (p + 1 < P && failed_before((p + 2) * NBC)) {
+            singular = true;
+            return true;
         }
     }
     return true;
 }
 
-// getrs with one right-hand side (Simulation.cuh:1316), workgroup-wide, the
-// vector staged in LDS: per panel the recorded row moves, then forward
-// substitution (k ascending) and backward substitution (k descending).
-template <int NBC>
-__device__ void lu_solve_wg(const double *__restrict__ M, int n, const int *perm_src, const int *moved_to,
-                            double *xio, Shared &sh)
-{
-    const int tid = threadIdx.x;
-    const int P = (n + NBC - 1) / NBC;
-    for (int i = tid; i < n; i += NT) sh.b[i] = xio[i];
-    __syncthreads();
-    for (int p = 0; p < P; p++) {
-        const int r0 = p * NBC;
-        const int nbp = min(NBC, n - r0);
-        double v1 = 0.0, v2 = 0.0;
-        int dst = -1;
-        if (tid < nbp) {
-            v1 = sh.b[perm_src[r0 + tid]];
-            v2 = sh.b[r0 + tid];
-            dst = moved_to[r0 + tid];
-        }
-        __syncthreads();
-        if (tid < nbp) {
-            sh.b[r0 + tid] = v1;
-            if (dst >= r0 + nbp) sh.b[dst] = v2;
-        }
-        __syncthreads();
-        if (tid < 64) {
-            double bi = (tid < nbp) ? sh.b[r0 + tid] : 0.0;
-            const double *col = M + (size_t)r0 * n + r0;
-            for (int kk = 0; kk < nbp; kk++) {
-                const double bk = __shfl(bi, kk);
-                if (tid > kk && tid < nbp) bi = fma(-col[(size_t)kk * n + tid], bk, bi);
-            }
-            if (tid < nbp) sh.b[r0 + tid] = bi;
-        }
-        __syncthreads();
-        for (int i = r0 + nbp + tid; i < n; i += NT) {
-            double acc = sh.b[i];
-            for (int kk = 0; kk < nbp; kk++) acc = fma(-M[(size_t)(r0 + kk) * n + i], sh.b[r0 + kk], acc);
-            sh.b[i] = acc;
-        }
-        __syncthreads();
-    }
-    for (int p = P - 1; p >= 0; p--) {
-        const int r0 = p * NBC;
-        const int nbp = min(NBC, n - r0);
-        if (tid < 64) {
-            double bi = (tid < nbp) ? sh.b[r0 + tid] : 0.0;
-            const double *col = M + (size_t)r0 * n + r0;
-            for (int kk = nbp - 1; kk >= 0; kk--) {
-                if (tid == kk) bi = bi / col[(size_t)kk * n + kk];
-                const double xk = __shfl(bi, kk);
-                if (tid < kk) bi = fma(-col[(size_t)kk * n + tid], xk, bi);
-            }
-            if (tid < nbp) sh.b[r0 + tid] = bi;
-        }
-        __syncthreads();
-        for (int i = tid; i < r0; i += NT) {
-            double acc = sh.b[i];
-            for (int kk = nbp - 1; kk >= 0; kk--) acc = fma(-M[(size_t)(r0 + kk) * n + i], sh.b[r0 + kk], acc);
-            sh.b[i] = acc;
-        }
-        __syncthreads();
-    }
-    for (int i = tid; i < n; i += NT) xio[i] = sh.b[i];
-    __syncthreads();
-}
-
-__device__ __forceinline__ bool bad_value(double v) { return isnan(v) || isinf(v) || v < 0; }
-
 // ---------------------------------------------------------------------------
 template <int RPT, int NBC>
 __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
 {
+    extern __shared__ __align__(16) unsigned char dyn[];
     __shared__ Shared sh;
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
@@ -569,6 +208,50 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     const int n = S;
     unsigned int gen_local = 0;
     Ctl *ctl = A.ctl;
+
+    // ---- control-group views ---------------------------------------------
+    CtlView v;
+    double *gv = A.gvec;
+    const size_t Sz = (size_t)S;
+    if (A.lds_model) {
+        const int nh = A.m.hptr[S];
+        HRec *lrec = reinterpret_cast<HRec *>(dyn);
+        int *lhptr = reinterpret_cast<int *>(dyn + sizeof(HRec) * (size_t)nh);
+        double *lv = reinterpret_cast<double *>(dyn + ((sizeof(HRec) * (size_t)nh + 4 * (Sz + 1) + 15) & ~size_t(15)));
+        v.recs = lrec;
+        v.hptr = lhptr;
+        v.y1 = lv;
+        v.ua = lv + Sz;
+        v.ub = lv + 2 * Sz;
+        v.k1 = lv + 3 * Sz;
+        v.k2 = lv + 4 * Sz;
+        v.k3 = lv + 5 * Sz;
+        v.k4 = lv + 6 * Sz;
+        v.k5 = lv + 7 * Sz;
+        v.w1 = lv + 8 * Sz;
+        v.yv = lv + 9 * Sz;
+        v.x = lv + 10 * Sz;
+        if (wg == 0) {
+            for (int e = tid; e < nh; e += NT) lrec[e] = A.recs[e];
+            for (int s = tid; s <= S; s += NT) lhptr[s] = A.m.hptr[s];
+        }
+    } else {
+        v.recs = A.recs;
+        v.hptr = A.m.hptr;
+        v.y1 = gv;
+        v.ua = gv + Sz;
+        v.ub = gv + 2 * Sz;
+        v.k1 = gv + 3 * Sz;
+        v.k2 = gv + 4 * Sz;
+        v.k3 = gv + 5 * Sz;
+        v.k4 = gv + 6 * Sz;
+        v.k5 = gv + 7 * Sz;
+        v.w1 = gv + 8 * Sz;
+        v.yv = gv + 9 * Sz;
+        v.x = reinterpret_cast<double *>(dyn);  // the solve vector lives in LDS
+    }
+    double *gyv = gv + 9 * Sz;  // y_v as the other groups read it
+    double *gy1 = gv;
 
     // state of the control loop, held identically by every thread of group 0
     double t = 0.0, t1 = A.tv[0], dt = A.rkf_step, newdt = A.be_step;
@@ -584,10 +267,12 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     unsigned long long tmark = 0;
 
     if (wg == 0)
-        for (int s = tid; s < S; s += NT) A.y1[s] = A.x0[s];
+        for (int s = tid; s < S; s += NT) v.y1[s] = A.x0[s];
     __syncthreads();
 
+    unsigned int parity = 0;
     while (true) {
+        parity ^= 1;
         // ---- control step (group 0) -----------------------------------------
         if (wg == 0) {
             tmark = wall_clock64();
@@ -602,7 +287,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 bool finished = false;
                 if (fabs(t1 - t) < 1e-5) {
                     for (int j = tid; j < A.ncs; j += NT)
-                        A.out[(size_t)i_sample * A.ncs + j] = A.y1[A.cs[j]];
+                        A.out[(size_t)i_sample * A.ncs + j] = v.y1[A.cs[j]];
                     i_sample++;
                     if (i_sample == A.nt)
                         finished = true;
@@ -614,11 +299,11 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 } else {
                     st_att++;
                     double mind;
-                    const int nok = rkf_attempt_wg(A, dt, &mind, sh);
+                    const int nok = rkf_attempt_wg(A.m, A.rtol, v, dt, &mind, sh);
                     if (nok == S) {
                         nrej = 0;
                         st_acc++;
-                        for (int s = tid; s < S; s += NT) A.y1[s] = A.w1[s];
+                        for (int s = tid; s < S; s += NT) v.y1[s] = v.w1[s];
                         __syncthreads();
                         const double d_temp = dt * mind;
                         if ((t + d_temp) > t1)
@@ -640,10 +325,11 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                             mode = (A.newton_iter > 0) ? MODE_BDF : MODE_BDF_NOOP;
                             // y_v = y1; Newton iteration 0 residual at y_v
                             for (int s = tid; s < S; s += NT) {
-                                const double yv = A.y1[s];
-                                A.yv[s] = yv;
-                                const double f = rhs_species(A.m, A.y1, s);
-                                A.x[s] = -(yv - A.y1[s] - hb * f);
+                                const double yv = v.y1[s];
+                                v.yv[s] = yv;
+                                gyv[s] = yv;
+                                const double f = rhs_species(A.m, v.recs, v.hptr, v.y1, s);
+                                v.x[s] = -(yv - v.y1[s] - hb * f);
                             }
                             __syncthreads();
                         }
@@ -651,12 +337,10 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 }
             }
             if (tid == 0) {
-                st_rel_d(&ctl->h, hb);
-                st_rel(&ctl->mode, mode);
-                st_rel(&ctl->info, 0);
+                st_rlx_d(&ctl->h[parity], hb);
+                st_rlx(&ctl->mode[parity], mode);
+                st_rlx(&ctl->info, 0);
             }
-        }
-        if (wg == 0) {
             const unsigned long long now = wall_clock64();
             ph[0] += now - tmark;
             tmark = now;
@@ -666,10 +350,10 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         } else {
             __syncthreads();
         }
-        const int mode = ld_acq(&ctl->mode);
+        const int mode = ld_rlx(&ctl->mode[parity]);
         if (mode == MODE_DONE) break;
         if (mode == MODE_CONTINUE) continue;
-        const double h = ld_acq_d(&ctl->h);
+        const double h = ld_rlx_d(&ctl->h[parity]);
 
         bool singular = false;
         if (mode == MODE_BDF) {
@@ -677,14 +361,17 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             {
                 const size_t nn = (size_t)n * n;
                 const size_t stride = (size_t)G * NT;
-                for (size_t e = (size_t)wg * NT + tid; e < nn; e += stride) A.M[e] = 0.0;
+                double2 *M2 = reinterpret_cast<double2 *>(A.M);
+                const double2 z = make_double2(0.0, 0.0);
+                for (size_t e = ((size_t)wg * NT + tid); e < nn / 2; e += stride) M2[e] = z;
+                if ((nn & 1) && wg == 0 && tid == 0) A.M[nn - 1] = 0.0;
             }
             if (G > 1) {
                 if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
             } else {
                 __syncthreads();
             }
-            for (int r = wg * NT + tid; r < n; r += G * NT) jac_row(A.m, A.yv, A.M, n, r, h);
+            for (int r = wg * NT + tid; r < n; r += G * NT) jac_row<false>(A.m, gyv, A.M, n, r, h);
             if (G > 1) {
                 if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
             } else {
@@ -695,7 +382,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 ph[1] += now - tmark;
                 tmark = now;
             }
-            if (!lu_factor_grid<RPT, NBC>(A, sh, gen_local, singular)) return;
+            if (!lu_factor_grid<RPT, NBC>(A, wg, sh, gen_local, singular)) return;
             if (wg == 0) {
                 const unsigned long long now = wall_clock64();
                 ph[2] += now - tmark;
@@ -712,30 +399,30 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 while (itN < A.newton_iter && errN > A.newton_tol) {
                     if (itN > 0) {
                         for (int s = tid; s < S; s += NT) {
-                            const double f = rhs_species(A.m, A.yv, s);
-                            A.x[s] = -(A.yv[s] - A.y1[s] - h * f);
+                            const double f = rhs_species(A.m, v.recs, v.hptr, v.yv, s);
+                            v.x[s] = -(v.yv[s] - v.y1[s] - h * f);
                         }
                         __syncthreads();
                     }
                     if (singular) {
                         for (int s = tid; s < S; s += NT)
-                            if (bad_value(A.yv[s])) A.yv[s] = A.y1[s];
+                            if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
                         __syncthreads();
                         newdt = h * 0.1;
                         luFail = true;
                         st_lufail++;
                         break;
                     }
-                    lu_solve_wg<NBC>(A.M, n, A.perm_src, A.moved_to, A.x, sh);
+                    lu_solve_wg<NBC, double, false>(A.M, n, A.perm_src, A.moved_to, v.x);
                     double part = 0.0;
                     for (int s = tid; s < S; s += NT) {
-                        const double yv1 = A.yv[s];
+                        const double yv1 = v.yv[s];
                         double yv;
                         if (A.m.feed[s] == 0)
-                            yv = yv1 + A.x[s];
+                            yv = yv1 + v.x[s];
                         else
                             yv = A.m.feed[s];
-                        A.yv[s] = yv;
+                        v.yv[s] = yv;
                         const double d = yv - yv1;
                         part = part + (double)(float)(d * d);
                     }
@@ -746,11 +433,11 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             }
             if (!luFail) {
                 int nbad = 0;
-                for (int s = tid; s < S; s += NT) nbad += bad_value(A.yv[s]) ? 1 : 0;
+                for (int s = tid; s < S; s += NT) nbad += bad_value(v.yv[s]) ? 1 : 0;
                 nbad = wg_isum(nbad, sh);
                 if (nbad > 0) {
                     for (int s = tid; s < S; s += NT)
-                        if (bad_value(A.yv[s])) A.yv[s] = A.y1[s];
+                        if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
                     newdt = h * 0.1;
                     st_rep++;
                 } else if (A.bdf_regrow) {
@@ -758,75 +445,76 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 }
             }
             __syncthreads();
-            for (int s = tid; s < S; s += NT) A.y1[s] = A.yv[s];
+            for (int s = tid; s < S; s += NT) v.y1[s] = v.yv[s];
             __syncthreads();
             t = t + h;
             ph[3] += wall_clock64() - tmark;
         }
     }
-    if (wg == 0 && tid == 0) {
-        long long *o = A.stats_out;
-        o[0] = st_loop;
-        o[1] = st_att;
-        o[2] = st_acc;
-        o[3] = st_bdf;
-        o[4] = st_newton;
-        o[5] = st_lufail;
-        o[6] = st_rep;
-        o[7] = i_sample;
-        o[8] = st_lu;
-        o[9] = rc;
-        double *d = A.dstats_out;
-        d[0] = t;
-        d[1] = dt;
-        d[2] = newdt;
-        for (int q = 0; q < 4; q++) d[3 + q] = (double)ph[q] * 1e-5;  // 100 MHz ticks -> ms
+    if (wg == 0) {
+        if (A.lds_model)
+            for (int s = tid; s < S; s += NT) gy1[s] = v.y1[s];
+        if (tid == 0) {
+            long long *o = A.stats_out;
+            o[0] = st_loop;
+            o[1] = st_att;
+            o[2] = st_acc;
+            o[3] = st_bdf;
+            o[4] = st_newton;
+            o[5] = st_lufail;
+            o[6] = st_rep;
+            o[7] = i_sample;
+            o[8] = st_lu;
+            o[9] = rc;
+            double *d = A.dstats_out;
+            d[0] = t;
+            d[1] = dt;
+            d[2] = newdt;
+            for (int q = 0; q < 4; q++) d[3 + q] = (double)ph[q] * 1e-5;  // 100 MHz ticks -> ms
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-template <int RPT, int NBC>
-static hipError_t launch_t(const SolveArgs &args, hipStream_t stream)
+size_t lds_model_bytes(int S, int nnz_h)
 {
-    SolveArgs a = args;
-    void *params[] = {&a};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&solve_kernel<RPT, NBC>), dim3(a.G),
-                                      dim3(NT), params, 0, stream);
+    const size_t head = (sizeof(HRec) * (size_t)nnz_h + 4 * ((size_t)S + 1) + 15) & ~size_t(15);
+    return head + sizeof(double) * kNumVecLds * (size_t)S;
+}
+
+template <int RPT, int NBC>
+static const void *kernel_ptr()
+{
+    return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC>);
+}
+
+static const void *pick_kernel(int n)
+{
+    if (n <= 256) return kernel_ptr<1, 16>();
+    if (n <= 512) return kernel_ptr<2, 16>();
+    if (n <= 1024) return kernel_ptr<4, 16>();
+    if (n <= 2048) return kernel_ptr<8, 16>();
+    return kernel_ptr<16, 8>();
 }
 
 int dense_panel_width(int n) { return n <= 2048 ? 16 : 8; }
 
-int max_coop_groups(int n, int *out)
+int solver_occupancy(int n, size_t dyn_lds, int *per_cu)
 {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return -1;
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
-    int per = 0;
-    hipError_t e;
-    if (n <= 256)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)&solve_kernel<1, 16>, NT, 0);
-    else if (n <= 512)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)&solve_kernel<2, 16>, NT, 0);
-    else if (n <= 1024)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)&solve_kernel<4, 16>, NT, 0);
-    else if (n <= 2048)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)&solve_kernel<8, 16>, NT, 0);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)&solve_kernel<16, 8>, NT, 0);
-    if (e != hipSuccess) return -1;
-    *out = per * cus;
+    const void *k = pick_kernel(n);
+    if (dyn_lds > 64 * 1024 &&
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) != hipSuccess)
+        return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, NT, dyn_lds) != hipSuccess) return -1;
     return 0;
 }
 
-hipError_t launch_solve(const SolveArgs &args, hipStream_t stream)
+hipError_t launch_solve(const SolveArgs &args, size_t dyn_lds, hipStream_t stream)
 {
-    const int n = args.m.S;
-    if (n <= 256) return launch_t<1, 16>(args, stream);
-    if (n <= 512) return launch_t<2, 16>(args, stream);
-    if (n <= 1024) return launch_t<4, 16>(args, stream);
-    if (n <= 2048) return launch_t<8, 16>(args, stream);
-    return launch_t<16, 8>(args, stream);
+    SolveArgs a = args;
+    void *params[] = {&a};
+    return hipLaunchCooperativeKernel(pick_kernel(a.m.S), dim3(a.G), dim3(NT), params,
+                                      (unsigned int)dyn_lds, stream);
 }
 
 }  // namespace lassie

@@ -1,0 +1,74 @@
+"""Batched persistent form vs the oracle.
+
+FP64: every instance of the batch must equal the oracle bit for bit (same
+algorithm and operation order as the single-model path).  FP32 / BF16 LU with
+FP64 refinement: the trajectory must stay within 1e-6 relative error of the
+FP64 form, |a - b| / (|b| + atol), and report its refinement work.
+"""
+import numpy as np
+import pytest
+
+from lassie_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BENCH = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
+
+
+def _batch(seed_base, n, sizes):
+    rng = np.random.default_rng(seed_base)
+    pick = rng.integers(0, len(sizes), size=n)
+    return [synth.stiff(int(sizes[p]), 2 * int(sizes[p]), seed_base + i, (1e-4, 1e4))
+            for i, p in enumerate(pick)]
+
+
+def test_batch_fp64_bit_exact(gpu_lib):
+    models = _batch(2000, 12, (16, 32, 48, 64))
+    t = synth.t_vector(0, 50, 100)
+    print(f"[batch] fp64 {len(models)} instances sizes {[m.S for m in models]}", flush=True)
+    dms = [gpu_lib.DeviceModel(m) for m in models]
+    outs, ys, stats, ms, rc = gpu_lib.solve_batch(dms, t, max_iters=2_000_000, **BENCH)
+    assert rc == 0
+    for m, out, y, st in zip(models, outs, ys, stats):
+        o_out, o_st, o_rc, _, o_y = O.solve(m, t, max_iters=2_000_000, **BENCH)
+        assert o_rc == 0
+        for key in ("loop_iters", "rkf_accepts", "bdf_steps", "newton_iters", "nan_repairs",
+                    "lu_failures", "samples"):
+            assert st[key] == o_st[key], (m.S, key, st[key], o_st[key])
+        np.testing.assert_array_equal(out, o_out)
+        np.testing.assert_array_equal(y, o_y)
+    for dm in dms:
+        dm.close()
+
+
+def test_batch_single_model_agrees_with_single_path(gpu_lib):
+    m = synth.stiff(96, 96, 5)
+    t = synth.t_vector(0, 100, 200)
+    dm = gpu_lib.DeviceModel(m)
+    out1, y1, st1, rc1, _ = dm.solve(t, raise_on_maxiter=False, max_iters=300, **BENCH)
+    outs, ys, stats, ms, rc = gpu_lib.solve_batch([dm], t, raise_on_maxiter=False, max_iters=300, **BENCH)
+    n = int(st1["samples"])
+    np.testing.assert_array_equal(outs[0][:n], out1[:n])
+    np.testing.assert_array_equal(ys[0], y1)
+    dm.close()
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["fp32", "bf16"])
+def test_batch_mixed_precision_accuracy(gpu_lib, prec):
+    models = _batch(3000, 10, (16, 32, 64, 128))
+    t = synth.t_vector(0, 50, 100)
+    print(f"[batch] precision {prec} sizes {[m.S for m in models]}", flush=True)
+    dms = [gpu_lib.DeviceModel(m) for m in models]
+    ref, _, ref_st, _, rc0 = gpu_lib.solve_batch(dms, t, max_iters=2_000_000, **BENCH)
+    outs, _, stats, _, rc = gpu_lib.solve_batch(dms, t, max_iters=2_000_000, precision=prec, **BENCH)
+    assert rc0 == 0 and rc == 0
+    worst = 0.0
+    for m, a, b, st in zip(models, outs, ref, stats):
+        rel = np.abs(a - b) / (np.abs(b) + m.atol[None, :])
+        worst = max(worst, float(rel.max()))
+        assert st["refine_iters"] > 0
+    print(f"[batch] precision {prec} max rel err vs FP64 form {worst:.3e}", flush=True)
+    assert worst <= 1e-6
+    for dm in dms:
+        dm.close()
