@@ -116,8 +116,8 @@ void lassie_opts_default(lassie_opts *o)
     o->max_rkf_retries = 0;
     o->precision = LASSIE_PREC_FP64;
     o->max_iters = 100000000LL;
-    o->refine_tol = 1e-10;
-    o->refine_max_iter = 8;
+    o->refine_tol = 1e-14;
+    o->refine_max_iter = 4;
 }
 
 int lassie_abi_version(void) { return LASSIE_ABI_VERSION; }
@@ -351,7 +351,9 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     int per_cu = 0;
     if (lassie::solver_occupancy(S, dyn, &per_cu) != 0 || per_cu < 1)
         return fail(LASSIE_ERR_HIP, "cannot size the cooperative launch");
-    const int G = std::max(1, std::min({P, cus * per_cu, 64}));
+    // opts.reserved bits 8..15 cap the number of groups (diagnostics)
+    const int gcap = (o.reserved >> 8) & 0xff;
+    const int G = std::max(1, std::min({P, cus * per_cu, gcap ? gcap : 64}));
     lassie::SolveArgs a{};
     a.m.S = S;
     a.m.R = m->R;

@@ -9,8 +9,8 @@
 //   FP32 / BF16      M is rounded to FP32 and factored with the same pivoting
 //                    in FP32 (BF16: trailing-update operands rounded to BF16,
 //                    FP32 accumulation); each Newton correction is refined in
-//                    FP64 against the FP64 M until ||r||_inf <= refine_tol *
-//                    ||b||_inf, else the step falls back to the FP64 LU.
+//                    FP64 against the FP64 M to a componentwise backward error
+//                    of refine_tol, else the step falls back to the FP64 LU.
 #include "batch.h"
 #include "device_common.h"
 #include "wg_common.h"
@@ -141,36 +141,37 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
 }
 
 // Mixed-precision Newton correction: solve M d = rhs with the FP32 factors and
-// FP64 residuals r = rhs - M64 x.  Returns true when the refinement met the
-// tolerance; x (double, LDS) holds the result.
+// FP64 residuals r = rhs - M64 x, until every component meets the
+// componentwise backward-error test |r_i| <= tol * ((|M64| |x|)_i + |rhs_i|)
+// (Oettli-Prager), so small components are as accurate as large ones.
+// Returns true on convergence; x (double, LDS) holds the result.
 __device__ bool refine_solve(const double *__restrict__ M64, const float *__restrict__ M32, int n, const int *perm_src,
                              const int *moved_to, const double *rhs, double *x, float *b32, double tol, int max_it,
                              long long &iters, Shared &sh)
 {
     const int tid = threadIdx.x;
-    double bn = 0.0;
     for (int i = tid; i < n; i += NT) {
         x[i] = 0.0;
         b32[i] = (float)rhs[i];
-        bn = fmax(bn, fabs(rhs[i]));
     }
-    bn = -wg_fmin(-bn, sh);
+    __syncthreads();
     for (int it = 0; it < max_it; it++) {
         iters++;
         lu_solve_wg<kBatchNB, float, true>(M32, n, perm_src, moved_to, b32);
         for (int i = tid; i < n; i += NT) x[i] = x[i] + (double)b32[i];
         __syncthreads();
-        double rn = 0.0;
+        int bad = 0;
         for (int i = tid; i < n; i += NT) {
             const double *row = M64 + (size_t)i * n;
-            double acc = rhs[i];
-            for (int j = 0; j < n; j++) acc = fma(-row[j], x[j], acc);
+            double acc = rhs[i], mag = fabs(rhs[i]);
+            for (int j = 0; j < n; j++) {
+                acc = fma(-row[j], x[j], acc);
+                mag = fma(fabs(row[j]), fabs(x[j]), mag);
+            }
             b32[i] = (float)acc;
-            rn = fmax(rn, fabs(acc));
+            bad |= !(fabs(acc) <= tol * mag);
         }
-        rn = -wg_fmin(-rn, sh);
-        if (rn <= tol * bn) return true;
-        if (!(rn == rn)) return false;  // NaN residual: give up at once
+        if (__syncthreads_or(bad) == 0) return true;
     }
     return false;
 }

@@ -161,6 +161,12 @@ void orc_newton_matrix(int S, double h, double *J)
  * Row-major, in place.  Returns 0, or k+1 for the first exactly-zero pivot
  * (elimination stops there; the caller treats the matrix as singular,
  * Simulation.cuh:1309-1312). */
+/* 1: the trailing update rounds the product and the difference separately
+ * (a - l*u) instead of one fma: an equally valid FP64 LU, used to measure how
+ * far two FP64 solvers' trajectories drift apart under this controller. */
+static int g_lu_two_roundings = 0;
+void orc_set_lu_variant(int two_roundings) { g_lu_two_roundings = two_roundings; }
+
 int orc_lu(int n, double *A, int32_t *piv)
 {
     for (int k = 0; k < n; k++) {
@@ -185,7 +191,14 @@ int orc_lu(int n, double *A, int32_t *piv)
             const double l = A[(size_t)i * n + k];
             double *ri = A + (size_t)i * n;
             const double *rk = A + (size_t)k * n;
-            for (int j = k + 1; j < n; j++) ri[j] = fma(-l, rk[j], ri[j]);
+            if (g_lu_two_roundings) {
+                for (int j = k + 1; j < n; j++) {
+                    const double prod = l * rk[j];
+                    ri[j] = ri[j] - prod;
+                }
+            } else {
+                for (int j = k + 1; j < n; j++) ri[j] = fma(-l, rk[j], ri[j]);
+            }
         }
     }
     return 0;

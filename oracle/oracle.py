@@ -73,6 +73,8 @@ def lib():
         L.orc_solve.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcOpts), _f64p, C.c_int, _i32p,
                                 C.c_int, _f64p, _f64p, _f64p, C.c_int64, C.POINTER(OrcStats)]
         L.orc_solve.restype = C.c_int
+        L.orc_set_lu_variant.argtypes = [C.c_int]
+        L.orc_set_lu_variant.restype = None
         _lib = L
     return _lib
 
@@ -182,3 +184,10 @@ def solve(m, t, cs=None, progress_cap=0, **kw):
     d = st.as_dict()
     n_prog = min(progress_cap, d["loop_iters"])
     return out, d, rc, prog[:n_prog], yf
+
+
+def set_lu_variant(two_roundings: bool):
+    """Switch the oracle's LU trailing update between fma (the specified
+    arithmetic) and separately rounded multiply/subtract (another valid FP64
+    LU, for measuring the FP64 noise floor of trajectories)."""
+    lib().orc_set_lu_variant(1 if two_roundings else 0)

@@ -1,0 +1,117 @@
+"""The LASSIE command line: argument and input-file errors end with the
+reference's exit codes before any GPU work (simulator.cu:61-207,
+Simulation.cuh:373-391, 1006-1011); on a GPU box the Solution file equals the
+oracle's trajectory printed the reference's way ("%.4e\\t")."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from lassie_amd import synth
+from oracle import oracle as O
+from tests import cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "lassie_amd", "LASSIE")
+
+
+def run(*args):
+    p = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=120)
+    rc = p.returncode
+    return (rc - 256 if rc > 127 else rc), p.stdout + p.stderr
+
+
+def model_folder(tmp_path, m=None, name="model", t=None, extra=None, cs=None):
+    m = m or synth.small_nonstiff(0)
+    d = tmp_path / name
+    synth.write_folder(m, str(d), synth.t_vector(0, 2, 5) if t is None else t, cs=cs, extra=extra)
+    return d
+
+
+def test_argument_errors():
+    assert run()[0] == -3
+    assert run("x")[0] == -2
+
+
+def test_missing_input_folder(tmp_path):
+    rc, out = run(str(tmp_path / "nope"), str(tmp_path / "out"))
+    assert rc == -1 and "input folder does not exist" in out
+
+
+def test_missing_mandatory_files(tmp_path):
+    d = model_folder(tmp_path)
+    os.remove(d / "c_vector")
+    os.remove(d / "M_0")
+    rc, out = run(str(d), str(tmp_path / "out"))
+    assert rc == -4
+    assert "kinetic constants vector" in out and "initial conditions" in out
+
+
+def test_bad_output_and_precision_flags(tmp_path):
+    d = model_folder(tmp_path)
+    assert run(str(d), "-v1")[0] == -2
+    assert run(str(d), str(tmp_path / "o"), "-p")[0] == -5
+    assert run(str(d), str(tmp_path / "o"), "-p", "half")[0] == -6
+
+
+def test_modelkind_and_volume_errors(tmp_path):
+    d = model_folder(tmp_path, extra={"modelkind": "sideways"})
+    assert run(str(d), str(tmp_path / "o"))[0] == -7
+    d2 = model_folder(tmp_path, name="m2", extra={"modelkind": "stochastic"})
+    assert run(str(d2), str(tmp_path / "o2"))[0] == -8
+
+
+def test_stochastic_rejects_three_reactants(tmp_path):
+    m = cases.from_reactions(3, [({0: 1, 1: 1, 2: 1}, {})], [1.0], [1.0, 1.0, 1.0])
+    d = model_folder(tmp_path, m, extra={"modelkind": "stochastic", "volume": "1e-15"})
+    assert run(str(d), str(tmp_path / "o"))[0] == -12
+
+
+def test_output_folder_and_sampling_file_are_created(tmp_path):
+    d = model_folder(tmp_path, cs=[3, 1])
+    out = tmp_path / "out"
+    run(str(d), str(out))
+    assert (out / "output").is_dir()
+    assert (out / "output" / "SpeciesSampling").read_text() == "Saving species' dynamics: \n3; 1; \n"
+
+
+def _fmt(t, out):
+    return "".join("%.4e\t" % ti + "".join("%.4e\t" % v for v in row) + "\n" for ti, row in zip(t, out))
+
+
+@pytest.mark.gpu
+def test_solution_file_matches_oracle(tmp_path, gpu_lib):
+    m = synth.small_nonstiff(0)
+    t = synth.t_vector(0, 10, 100)
+    d = model_folder(tmp_path, m, t=t, cs=[0, 5, 31])
+    print(f"[cli] 32x48 model, faithful defaults", flush=True)
+    rc, log = run(str(d), str(tmp_path / "out"))
+    assert rc == 0, log
+    o_out, o_st, o_rc, prog, _ = O.solve(m, t, cs=np.array([0, 5, 31]), progress_cap=1 << 20)
+    assert (tmp_path / "out" / "output" / "Solution").read_text() == _fmt(t, o_out)
+    lines = (tmp_path / "out" / "output" / "ProgressSimulation").read_text().splitlines()
+    assert lines == ["%.4e" % v for v in prog]
+
+
+@pytest.mark.gpu
+def test_stochastic_conversion_matches_oracle(tmp_path, gpu_lib):
+    m = cases.mixed()
+    vol, avo = 1e-15, 6.022e23
+    m_num = synth.model_from_dense(m.L, m.Rm, m.k, m.x0 * vol * avo)
+    t = synth.t_vector(0, 3, 7)
+    d = model_folder(tmp_path, m_num, t=t, extra={"modelkind": "stochastic", "volume": "%.17g" % vol})
+    rc, log = run(str(d), str(tmp_path / "out"))
+    assert rc == 0, log
+    # Simulation.cuh:983-1031: x0 / (V N_A); bimolecular k * V N_A; homodimer k * V N_A / 2
+    conv = vol * avo
+    k = m.k.copy()
+    for r in range(m.R):
+        distinct = int(np.sum(m.L[r] > 0))
+        if np.any(m.L[r] >= 2):
+            k[r] = (k[r] * conv) / 2.0
+        elif distinct > 1:
+            k[r] = k[r] * conv
+    m_conc = synth.model_from_dense(m.L, m.Rm, k, (m.x0 * vol * avo) / conv)
+    o_out, _, o_rc, _, _ = O.solve(m_conc, t)
+    assert (tmp_path / "out" / "output" / "Solution").read_text() == _fmt(t, o_out * (vol * avo))
