@@ -127,6 +127,45 @@ int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lass
                        const double *t, int32_t nt, double *out, double *y_final, lassie_stats *stats,
                        double *elapsed_ms);
 
+/* Replaces the rate constants (c_vector) of an uploaded model, k[R].  A tape
+ * recorded before the change can no longer be differentiated.  No
+ * counterpart in the reference (its rates are fixed per process,
+ * Simulation.cuh:397-418); used by the differentiation form. */
+int lassie_model_set_rates(lassie_model *model, const double *k);
+
+/* ---- Differentiation form --------------------------------------------------
+ * Gradients of a loss on the sampled trajectories with respect to log k and
+ * x0, by a discrete adjoint through the RKF45 and BDF1 steps of the loop
+ * (Simulation.cuh:632-925, kernels.cuh:68-199, 289-300, 393-477, 512-521).
+ * No counterpart in the reference.
+ *
+ * lassie_forward_batch runs the batched form (FP64, same trajectories as
+ * lassie_solve_batch) and records, per model, every state-changing step (size
+ * and kind) and the state before every 32nd step, in a tape of at most
+ * max_steps steps per model (LASSIE_ERR_NOMEM when a model takes more).  The
+ * models must stay alive, with unchanged rates, until the tape is destroyed.
+ * out receives nt x S_i values per model, model after model. */
+typedef struct lassie_tape lassie_tape;
+int lassie_forward_batch(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                         const double *t, int32_t nt, int64_t max_steps, double *out, lassie_stats *stats,
+                         double *elapsed_ms, lassie_tape **tape);
+
+/* Reverse pass: grad_out holds dL/d(out) in the layout of out; grad_log_k
+ * receives dL/dlog k (R_i per model, model after model), grad_x0 (may be NULL)
+ * dL/dx0 (S_i per model).  The step sequence is the recorded one (step sizes
+ * and Newton iteration counts held fixed). */
+int lassie_adjoint_batch(lassie_tape *tape, const double *grad_out, double *grad_log_k, double *grad_x0,
+                         double *elapsed_ms);
+
+/* Steps recorded for one model and the largest Newton iteration count of its
+ * BDF1 steps; lassie_tape_read copies the record out: dt[n_steps], kind[n_steps]
+ * (-1 accepted RKF45 step, -2 BDF1 step whose LU failed, else the BDF1 step's
+ * Newton iterations), sample_step[nt] (steps taken when each sample was
+ * stored).  Any output may be NULL. */
+int lassie_tape_info(const lassie_tape *tape, int32_t model, int64_t *n_steps, int32_t *max_newton);
+int lassie_tape_read(const lassie_tape *tape, int32_t model, double *dt, int32_t *kind, int32_t *sample_step);
+int lassie_tape_destroy(lassie_tape *tape);
+
 /* Host helper: CSR arrays from dense R x S stoichiometry (int16, row-major, as
  * Reader<T>::readMatrix returns them, Reader.h:104-124), in the entry order
  * of create_sparse2 / create_sparse (Simulation.cuh:1079-1130).  Output

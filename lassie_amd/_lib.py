@@ -66,8 +66,9 @@ class Stats(C.Structure):
 
 # every symbol include/lassie.h declares
 EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy", "lassie_solve",
-           "lassie_solve_batch", "lassie_csr_from_dense", "lassie_device_count",
-           "lassie_last_error", "lassie_abi_version")
+           "lassie_solve_batch", "lassie_model_set_rates", "lassie_forward_batch",
+           "lassie_adjoint_batch", "lassie_tape_info", "lassie_tape_read", "lassie_tape_destroy",
+           "lassie_csr_from_dense", "lassie_device_count", "lassie_last_error", "lassie_abi_version")
 
 
 class LassieError(RuntimeError):
@@ -97,6 +98,14 @@ def lib():
                                          C.POINTER(C.c_double)]
         L.lassie_csr_from_dense.argtypes = [C.c_int32, C.c_int32, _i16p, _i16p, _i32p, _i32p,
                                             _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]
+        L.lassie_model_set_rates.argtypes = [C.c_void_p, _f64p]
+        L.lassie_forward_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts), _f64p,
+                                           C.c_int32, C.c_int64, _f64p, C.POINTER(Stats),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_void_p)]
+        L.lassie_adjoint_batch.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, C.POINTER(C.c_double)]
+        L.lassie_tape_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), _i32p]
+        L.lassie_tape_read.argtypes = [C.c_void_p, C.c_int32, _f64p, _i32p, _i32p]
+        L.lassie_tape_destroy.argtypes = [C.c_void_p]
         L.lassie_device_count.argtypes = [_i32p]
         L.lassie_last_error.restype = C.c_char_p
         L.lassie_abi_version.restype = C.c_int
@@ -174,6 +183,13 @@ class DeviceModel:
             lib().lassie_model_destroy(self.h)
             self.h = None
 
+    def set_rates(self, k):
+        """Replace the rate constants (lassie_model_set_rates)."""
+        k = np.ascontiguousarray(k, np.float64)
+        if k.shape != (self.R,):
+            raise ValueError(f"k must have shape ({self.R},)")
+        _check(lib().lassie_model_set_rates(self.h, _p(k, _f64p)))
+
     def __del__(self):
         try:
             self.close()
@@ -223,3 +239,78 @@ def solve_batch(models, t, raise_on_maxiter=True, **opts):
         o_off += t.size * S
         y_off += S
     return outs, ys, [s.as_dict() for s in st], ms.value, rc
+
+
+class Tape:
+    """Step record of one forward pass of the differentiation form
+    (lassie_forward_batch); keeps its models alive until closed."""
+
+    def __init__(self, h, models, nt):
+        self.h = h
+        self.models = list(models)
+        self.nt = nt
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lassie_tape_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self, i):
+        n = C.c_int64(0)
+        mx = np.zeros(1, np.int32)
+        _check(lib().lassie_tape_info(self.h, i, C.byref(n), _p(mx, _i32p)))
+        return n.value, int(mx[0])
+
+    def read(self, i):
+        """(dt[n], kind[n], sample_step[nt]) of model i."""
+        n, _ = self.info(i)
+        dt = np.zeros(max(n, 1))
+        kind = np.zeros(max(n, 1), np.int32)
+        ss = np.zeros(self.nt, np.int32)
+        _check(lib().lassie_tape_read(self.h, i, _p(dt, _f64p), _p(kind, _i32p), _p(ss, _i32p)))
+        return dt[:n], kind[:n], ss
+
+    def adjoint(self, grad_outs):
+        """grad_outs: list of dL/d(out_i) [nt, S_i].  Returns (list of dL/dlog k
+        [R_i], list of dL/dx0 [S_i], device ms)."""
+        g = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).ravel() for x in grad_outs]))
+        gk = np.zeros(sum(m.R for m in self.models))
+        gx = np.zeros(sum(m.S for m in self.models))
+        ms = C.c_double(0.0)
+        _check(lib().lassie_adjoint_batch(self.h, _p(g, _f64p), _p(gk, _f64p), _p(gx, _f64p), C.byref(ms)))
+        gks, gxs = [], []
+        ko = xo = 0
+        for m in self.models:
+            gks.append(gk[ko:ko + m.R])
+            gxs.append(gx[xo:xo + m.S])
+            ko += m.R
+            xo += m.S
+        return gks, gxs, ms.value
+
+
+def forward_batch(models, t, max_steps=1 << 16, **opts):
+    """Forward pass of the differentiation form.  Returns (list of out[nt, S_i],
+    list of stats dicts, device ms, Tape)."""
+    t = np.ascontiguousarray(t, np.float64)
+    n = len(models)
+    sizes = [dm.S for dm in models]
+    out = np.full(t.size * sum(sizes), np.nan)
+    st = (Stats * n)()
+    hs = (C.c_void_p * n)(*[dm.h for dm in models])
+    ms = C.c_double(0.0)
+    o = default_opts(**opts)
+    th = C.c_void_p()
+    _check(lib().lassie_forward_batch(hs, n, C.byref(o), _p(t, _f64p), t.size, max_steps, _p(out, _f64p), st,
+                                      C.byref(ms), C.byref(th)))
+    outs = []
+    off = 0
+    for S in sizes:
+        outs.append(out[off:off + t.size * S].reshape(t.size, S))
+        off += t.size * S
+    return outs, [s.as_dict() for s in st], ms.value, Tape(th, models, t.size)

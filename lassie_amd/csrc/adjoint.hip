@@ -1,0 +1,487 @@
+// Reverse pass of the differentiation form: the discrete adjoint of LASSIE's
+// integration loop (Simulation.cuh:632-925) with respect to log k and x0.
+//
+// The forward pass (batch_kernel with a step record) stores the size and kind
+// of every state-changing step and the state before every kCkptEvery-th step.
+// Here one workgroup takes one instance from a work queue and walks its
+// segments from the last: it replays the segment's steps from the checkpoint
+// (same code as the forward, so the states are bit-identical), then runs the
+// steps backwards, adding dL/d(sample) at each sampled state.
+//
+//   RKF45 step: y' = y + 25/216 k1 + 1408/2565 k3 + 2197/4104 k4 - 1/5 k5,
+//     k_i = dt f(u_i) (rk_Fehlberg1..5 and _agg, kernels.cuh:68-182).
+//   BDF1 step: modified Newton with M = I - h Jq(y), Jq the analytic Jacobian
+//     of createJac_ (kernels.cuh:219-271) at the step's start state, run for
+//     the recorded number of iterations: r_j = -(v_j - y - h f(v_j)),
+//     x_j = M^-1 r_j, v_{j+1} = v_j + x_j, v_0 = y; entries ending NaN,
+//     infinite or negative are reset to y (aggiornaNaN, kernels.cuh:512-521).
+//     Its adjoint, per iteration from the last: z = M^-T vbar;
+//     ybar += h d(z^T Jq(y) x_j)/dy + z; vbar += -z + h Jf(v_j)^T z;
+//     log-k gradient += h d(z^T Jq x_j)/dlogk + h (df/dlogk)^T z.
+//   Feed species hold the feed value in every stage and iterate.
+//
+// Mass action: rate_r = k_r prod x_a^e_a, f = H rate, df/dlogk_r = H[:,r] rate_r.
+// Jf^T z and (df/dlogk)^T z go through c = H^T z (reaction-major H) and a
+// species-major list of reactant entries, so every sum has one owner thread
+// and a fixed order; there are no atomics.
+#include "adjoint.h"
+#include "batch.h"
+#include "device_common.h"
+#include "wg_common.h"
+#include "wg_lu.h"
+
+namespace lassie {
+
+__device__ __forceinline__ double rate_r(const AdjInst &I, const double *x, int r)
+{
+    double p = I.k[r];
+    for (int j = I.lptr[r]; j < I.lptr[r + 1]; j++) p *= mypow(x[I.lsp[j]], I.lexp[j]);
+    return p;
+}
+
+// d rate_r / d x_g for the reactant entry jg of reaction r
+__device__ __forceinline__ double drate(const AdjInst &I, const double *x, int r, int jg)
+{
+    const int e = I.lexp[jg];
+    double p = I.k[r] * (double)e * mypow(x[I.lsp[jg]], e - 1);
+    for (int j = I.lptr[r]; j < I.lptr[r + 1]; j++)
+        if (j != jg) p *= mypow(x[I.lsp[j]], I.lexp[j]);
+    return p;
+}
+
+// createJac_'s term of reaction r in column lsp[jg], without the net
+// coefficient: k e_g x_g^(e_g - 1) prod_{a != g} x_a
+__device__ __forceinline__ double qterm(const AdjInst &I, const double *x, int r, int jg)
+{
+    const int e = I.lexp[jg];
+    double p = I.k[r] * (double)e * mypow(x[I.lsp[jg]], e - 1);
+    for (int j = I.lptr[r]; j < I.lptr[r + 1]; j++)
+        if (j != jg) p *= x[I.lsp[j]];
+    return p;
+}
+
+// d qterm(r, jg) / d x_{lsp[jb]}
+__device__ __forceinline__ double dqterm(const AdjInst &I, const double *x, int r, int jg, int jb)
+{
+    const int e = I.lexp[jg];
+    double p;
+    if (jb == jg) {
+        if (e < 2) return 0.0;
+        p = I.k[r] * (double)e * (double)(e - 1) * mypow(x[I.lsp[jg]], e - 2);
+        for (int j = I.lptr[r]; j < I.lptr[r + 1]; j++)
+            if (j != jg) p *= x[I.lsp[j]];
+    } else {
+        p = I.k[r] * (double)e * mypow(x[I.lsp[jg]], e - 1);
+        for (int j = I.lptr[r]; j < I.lptr[r + 1]; j++)
+            if (j != jg && j != jb) p *= x[I.lsp[j]];
+    }
+    return p;
+}
+
+// c = H^T z, one thread per reaction
+__device__ void wg_ht(const AdjInst &I, const double *z, double *c)
+{
+    for (int r = threadIdx.x; r < I.R; r += NT) {
+        double a = 0.0;
+        for (int e = I.htptr[r]; e < I.htptr[r + 1]; e++) a += (double)I.htcoef[e] * z[I.htsp[e]];
+        c[r] = a;
+    }
+    __syncthreads();
+}
+
+// out += scale Jf(x)^T z ; gth += scale (df/dlogk)(x)^T z
+__device__ void wg_vjp_f(const AdjInst &I, const double *x, const double *z, double scale, double *out, double *gth,
+                         double *c)
+{
+    wg_ht(I, z, c);
+    for (int g = threadIdx.x; g < I.S; g += NT) {
+        double a = 0.0;
+        for (int e = I.ltptr[g]; e < I.ltptr[g + 1]; e++) {
+            const int jg = I.ltent[e];
+            const int r = I.lrx[jg];
+            if (c[r] != 0.0) a += c[r] * drate(I, x, r, jg);
+        }
+        out[g] += scale * a;
+    }
+    for (int r = threadIdx.x; r < I.R; r += NT)
+        if (c[r] != 0.0) gth[r] += scale * c[r] * rate_r(I, x, r);
+    __syncthreads();
+}
+
+// phi = z^T Jq(y) x: gy += scale dphi/dy ; gth += scale dphi/dlogk
+__device__ void wg_vjp_jq(const AdjInst &I, const double *y, const double *z, const double *x, double scale,
+                          double *gy, double *gth, double *c)
+{
+    wg_ht(I, z, c);
+    for (int b = threadIdx.x; b < I.S; b += NT) {
+        double a = 0.0;
+        for (int e = I.ltptr[b]; e < I.ltptr[b + 1]; e++) {
+            const int jb = I.ltent[e];
+            const int r = I.lrx[jb];
+            if (c[r] == 0.0) continue;
+            double s2 = 0.0;
+            for (int jg = I.lptr[r]; jg < I.lptr[r + 1]; jg++) s2 += x[I.lsp[jg]] * dqterm(I, y, r, jg, jb);
+            a += c[r] * s2;
+        }
+        gy[b] += scale * a;
+    }
+    for (int r = threadIdx.x; r < I.R; r += NT) {
+        if (c[r] == 0.0) continue;
+        double th = 0.0;
+        for (int jg = I.lptr[r]; jg < I.lptr[r + 1]; jg++) th += qterm(I, y, r, jg) * x[I.lsp[jg]];
+        gth[r] += scale * c[r] * th;
+    }
+    __syncthreads();
+}
+
+// Solve M^T z = b with the factors of wg_lu_rm (P M = L U, rows of M in final
+// order, P the interchanges piv[0..n-1]): U^T w = b forward, L^T u = w
+// backward, in 64-row blocks (diagonal block by one wave with shuffles, the
+// rest by all threads), then z = P^T u (interchanges in reverse order).
+__device__ void lu_solve_t_wg(const double *__restrict__ M, int n, const int *piv, double *b)
+{
+    constexpr int TB = 64;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int NB = (n + TB - 1) / TB;
+    for (int B = 0; B < NB; B++) {
+        const int k0 = B * TB, nb = min(TB, n - k0);
+        if (tid < 64) {
+            double bi = lane < nb ? b[k0 + lane] : 0.0;
+            for (int kk = 0; kk < nb; kk++) {
+                const double w = __shfl(bi, kk) / M[(size_t)(k0 + kk) * n + k0 + kk];
+                if (lane == kk) bi = w;
+                if (lane > kk && lane < nb) bi -= M[(size_t)(k0 + kk) * n + k0 + lane] * w;
+            }
+            if (lane < nb) b[k0 + lane] = bi;
+        }
+        __syncthreads();
+        for (int i = k0 + nb + tid; i < n; i += NT) {
+            double a = b[i];
+            for (int kk = 0; kk < nb; kk++) a -= M[(size_t)(k0 + kk) * n + i] * b[k0 + kk];
+            b[i] = a;
+        }
+        __syncthreads();
+    }
+    for (int B = NB - 1; B >= 0; B--) {
+        const int k0 = B * TB, nb = min(TB, n - k0);
+        if (tid < 64) {
+            double bi = lane < nb ? b[k0 + lane] : 0.0;
+            for (int kk = nb - 1; kk >= 0; kk--) {
+                const double u = __shfl(bi, kk);
+                if (lane < kk) bi -= M[(size_t)(k0 + kk) * n + k0 + lane] * u;
+            }
+            if (lane < nb) b[k0 + lane] = bi;
+        }
+        __syncthreads();
+        for (int i = tid; i < k0; i += NT) {
+            double a = b[i];
+            for (int kk = 0; kk < nb; kk++) a -= M[(size_t)(k0 + kk) * n + i] * b[k0 + kk];
+            b[i] = a;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int k = n - 1; k >= 0; k--) {
+            const int p = piv[k];
+            if (p != k) {
+                const double t = b[k];
+                b[k] = b[p];
+                b[p] = t;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+struct AdjWork {
+    double *seg, *M, *V, *X;
+    double *ua, *ub, *k1, *k2, *k3, *k4, *k5, *w1;
+    double *lam, *yb, *vb, *ubar, *kb[6], *u[6], *c;
+    int *piv, *perm_src, *moved_to, *info, *rowcnt, *rows;
+};
+
+// BDF1 step of size h from y with `iters` modified-Newton iterations, the
+// arithmetic of batch_kernel's FP64 path: iterates in V (v_0 = y), raw
+// corrections in X, factors in W.M; the repaired final iterate in y_out.
+// Returns false when the LU fails.
+__device__ bool bdf_iterates(const DevModel &m, const AdjInst &I, const double *y, double h, int iters, double *y_out,
+                             AdjWork &W, double *s_z, Shared &sh)
+{
+    const int S = m.S;
+    const int tid = threadIdx.x;
+    for (int s = tid; s < S; s += NT) W.V[s] = y[s];
+    const size_t nn = (size_t)S * S;
+    for (size_t e = tid; e < nn; e += NT) W.M[e] = 0.0;
+    __syncthreads();
+    for (int r = tid; r < S; r += NT) jac_row<true>(m, y, W.M, S, r, h);
+    __syncthreads();
+    if (wg_lu_rm<double, false>(W.M, S, W.piv, W.perm_src, W.moved_to, W.rows, W.rowcnt, W.info, sh) != 0) return false;
+    for (int j = 0; j < iters; j++) {
+        const double *vj = W.V + (size_t)j * S;
+        double *vn = W.V + (size_t)(j + 1) * S;
+        double *xj = W.X + (size_t)j * S;
+        for (int s = tid; s < S; s += NT) {
+            const double f = rhs_species(m, I.recs, I.hptr, vj, s);
+            s_z[s] = -(vj[s] - y[s] - h * f);
+        }
+        __syncthreads();
+        lu_solve_wg<kBatchNB, double, true>(W.M, S, W.perm_src, W.moved_to, s_z);
+        for (int s = tid; s < S; s += NT) {
+            xj[s] = s_z[s];
+            vn[s] = (m.feed[s] == 0) ? vj[s] + s_z[s] : m.feed[s];
+        }
+        __syncthreads();
+    }
+    const double *vm = W.V + (size_t)iters * S;
+    for (int s = tid; s < S; s += NT) y_out[s] = bad_value(vm[s]) ? y[s] : vm[s];
+    __syncthreads();
+    return true;
+}
+
+__device__ void rkf_forward(const DevModel &m, const AdjInst &I, double rtol, const double *y, double dt, AdjWork &W,
+                            Shared &sh)
+{
+    CtlView v;
+    v.recs = I.recs;
+    v.hptr = I.hptr;
+    v.y1 = const_cast<double *>(y);
+    v.ua = W.ua;
+    v.ub = W.ub;
+    v.k1 = W.k1;
+    v.k2 = W.k2;
+    v.k3 = W.k3;
+    v.k4 = W.k4;
+    v.k5 = W.k5;
+    v.w1 = W.w1;
+    v.yv = nullptr;
+    v.x = nullptr;
+    double mind;
+    (void)rkf_attempt_wg(m, rtol, v, dt, &mind, sh);
+}
+
+__global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
+{
+    __shared__ Shared sh;
+    __shared__ double s_z[kBatchMaxS];
+    __shared__ int s_idx;
+    const int tid = threadIdx.x;
+    const int slot = blockIdx.x;
+    const int smax = A.s_max;
+    double *ws = A.ws + (size_t)slot * A.ws_stride;
+    AdjWork W;
+    {
+        double *p = ws;
+        W.seg = p;
+        p += (size_t)(kCkptEvery + 1) * smax;
+        W.M = p;
+        p += (size_t)smax * smax;
+        W.V = p;
+        p += (size_t)(A.it_max + 1) * smax;
+        W.X = p;
+        p += (size_t)(A.it_max + 1) * smax;
+        double **vecs[] = {&W.ua, &W.ub, &W.k1, &W.k2, &W.k3, &W.k4, &W.k5, &W.w1, &W.lam, &W.yb, &W.vb, &W.ubar};
+        for (double **q : vecs) {
+            *q = p;
+            p += smax;
+        }
+        for (int i = 1; i <= 5; i++) {
+            W.kb[i] = p;
+            p += smax;
+            W.u[i] = p;
+            p += smax;
+        }
+        W.c = p;
+        int *wi = A.wsi + (size_t)slot * A.wsi_stride;
+        const int Pmax = (smax + kBatchNB - 1) / kBatchNB;
+        W.piv = wi;
+        W.perm_src = wi + smax;
+        W.moved_to = wi + 2 * smax;
+        W.info = wi + 3 * smax;
+        W.rowcnt = wi + 3 * smax + 16;
+        W.rows = W.rowcnt + Pmax;
+    }
+
+    while (true) {
+        if (tid == 0) s_idx = atomicAdd(A.next, 1);
+        __syncthreads();
+        const int idx = s_idx;
+        __syncthreads();
+        if (idx >= A.n_inst) return;
+        const int inst = A.order[idx];
+        const AdjInst I = A.inst[inst];
+        const int S = I.S, R = I.R;
+        DevModel m;
+        m.S = S;
+        m.R = R;
+        m.lptr = I.lptr;
+        m.lsp = I.lsp;
+        m.lexp = I.lexp;
+        m.hptr = I.hptr;
+        m.hrx = I.hrx;
+        m.hk = I.hk;
+        m.feed = I.feed;
+        m.atol = I.atol;
+        const double *fd = I.feed;
+        const double *tdt = A.tape_dt + I.tape_off;
+        const int *tkind = A.tape_kind + I.tape_off;
+        const int *samp = A.tape_sample + (size_t)inst * A.nt;
+        const double *gout = A.grad_out + I.g_off;
+        double *gk = A.grad_logk + I.gk_off;
+        int status = 0;
+        for (int r = tid; r < R; r += NT) gk[r] = 0.0;
+        for (int s = tid; s < S; s += NT) W.lam[s] = 0.0;
+        __syncthreads();
+
+        const long long N = I.n_steps;
+        int isamp = A.nt - 1;
+        auto inject = [&](long long n) {
+            while (isamp >= 0 && samp[isamp] == n) {
+                for (int s = tid; s < S; s += NT) W.lam[s] += gout[(size_t)isamp * S + s];
+                isamp--;
+            }
+            __syncthreads();
+        };
+        const long long nseg = (N + kCkptEvery - 1) / kCkptEvery;
+        for (long long sg = nseg - 1; sg >= 0 && status == 0; sg--) {
+            const long long n0 = sg * kCkptEvery;
+            const long long n1 = min(N, n0 + kCkptEvery);
+            const double *ck = A.tape_ckpt + I.ckpt_off + sg * S;
+            for (int s = tid; s < S; s += NT) W.seg[s] = ck[s];
+            __syncthreads();
+            // states n0 .. n1-1 of this segment
+            for (long long n = n0; n < n1 - 1; n++) {
+                const double *y = W.seg + (size_t)(n - n0) * S;
+                double *yn = W.seg + (size_t)(n - n0 + 1) * S;
+                const int kind = tkind[n];
+                const double dt = tdt[n];
+                if (kind == -1) {
+                    rkf_forward(m, I, A.rtol, y, dt, W, sh);
+                    for (int s = tid; s < S; s += NT) yn[s] = W.w1[s];
+                    __syncthreads();
+                } else if (kind == -2) {
+                    for (int s = tid; s < S; s += NT) yn[s] = y[s];
+                    __syncthreads();
+                } else if (!bdf_iterates(m, I, y, dt, kind, yn, W, s_z, sh)) {
+                    status = 1;
+                    break;
+                }
+            }
+            if (status) break;
+            for (long long n = n1; n > n0; n--) {
+                inject(n);
+                const long long step = n - 1;
+                const int kind = tkind[step];
+                const double dt = tdt[step];
+                const double *y = W.seg + (size_t)(step - n0) * S;
+                if (kind == -2) continue;
+                if (kind == -1) {
+                    rkf_forward(m, I, A.rtol, y, dt, W, sh);
+                    const double *k1 = W.k1, *k2 = W.k2, *k3 = W.k3, *k4 = W.k4;
+                    for (int s = tid; s < S; s += NT) {
+                        W.u[1][s] = y[s];
+                        const bool fed = fd[s] != 0.0;
+                        W.u[2][s] = fed ? fd[s] : y[s] + k1[s] / 4.0;
+                        W.u[3][s] = fed ? fd[s] : y[s] + (3.0 * k1[s]) / 32.0 + (9.0 * k2[s]) / 32.0;
+                        W.u[4][s] = fed ? fd[s]
+                                        : y[s] + (1932.0 * k1[s]) / 2197.0 - (7200.0 * k2[s]) / 2197.0 +
+                                              (7296.0 * k3[s]) / 2197.0;
+                        W.u[5][s] = fed ? fd[s]
+                                        : y[s] + (439.0 * k1[s]) / 216.0 - (8.0 * k2[s]) + (3680.0 * k3[s]) / 513.0 -
+                                              (845.0 * k4[s]) / 4104.0;
+                        const double l = fed ? 0.0 : W.lam[s];
+                        W.yb[s] = l;
+                        W.kb[1][s] = (25.0 / 216.0) * l;
+                        W.kb[2][s] = 0.0;
+                        W.kb[3][s] = (1408.0 / 2565.0) * l;
+                        W.kb[4][s] = (2197.0 / 4104.0) * l;
+                        W.kb[5][s] = (-1.0 / 5.0) * l;
+                    }
+                    __syncthreads();
+                    for (int st = 5; st >= 1; st--) {
+                        for (int s = tid; s < S; s += NT) W.ubar[s] = 0.0;
+                        __syncthreads();
+                        wg_vjp_f(I, W.u[st], W.kb[st], dt, W.ubar, gk, W.c);
+                        for (int s = tid; s < S; s += NT) {
+                            if (fd[s] != 0.0) continue;
+                            const double ub = W.ubar[s];
+                            W.yb[s] += ub;
+                            if (st == 5) {
+                                W.kb[1][s] += (439.0 / 216.0) * ub;
+                                W.kb[2][s] += -8.0 * ub;
+                                W.kb[3][s] += (3680.0 / 513.0) * ub;
+                                W.kb[4][s] += (-845.0 / 4104.0) * ub;
+                            } else if (st == 4) {
+                                W.kb[1][s] += (1932.0 / 2197.0) * ub;
+                                W.kb[2][s] += (-7200.0 / 2197.0) * ub;
+                                W.kb[3][s] += (7296.0 / 2197.0) * ub;
+                            } else if (st == 3) {
+                                W.kb[1][s] += (3.0 / 32.0) * ub;
+                                W.kb[2][s] += (9.0 / 32.0) * ub;
+                            } else if (st == 2) {
+                                W.kb[1][s] += 0.25 * ub;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    for (int s = tid; s < S; s += NT) W.lam[s] = W.yb[s];
+                    __syncthreads();
+                } else {
+                    if (!bdf_iterates(m, I, y, dt, kind, W.w1, W, s_z, sh)) {
+                        status = 1;
+                        break;
+                    }
+                    const double *vm = W.V + (size_t)kind * S;
+                    for (int s = tid; s < S; s += NT) {
+                        double vb = W.lam[s], yb = 0.0;
+                        if (bad_value(vm[s])) {
+                            yb = vb;
+                            vb = 0.0;
+                        }
+                        W.vb[s] = vb;
+                        W.yb[s] = yb;
+                    }
+                    __syncthreads();
+                    for (int j = kind - 1; j >= 0; j--) {
+                        for (int s = tid; s < S; s += NT) {
+                            if (fd[s] != 0.0) W.vb[s] = 0.0;
+                            s_z[s] = W.vb[s];
+                        }
+                        __syncthreads();
+                        lu_solve_t_wg(W.M, S, W.piv, s_z);
+                        wg_vjp_jq(I, y, s_z, W.X + (size_t)j * S, dt, W.yb, gk, W.c);
+                        for (int s = tid; s < S; s += NT) {
+                            W.vb[s] -= s_z[s];
+                            W.yb[s] += s_z[s];
+                        }
+                        __syncthreads();
+                        wg_vjp_f(I, W.V + (size_t)j * S, s_z, dt, W.vb, gk, W.c);
+                    }
+                    for (int s = tid; s < S; s += NT) W.lam[s] = W.yb[s] + W.vb[s];
+                    __syncthreads();
+                }
+            }
+        }
+        if (status == 0) inject(0);
+        double *gx = A.grad_x0 + I.gx_off;
+        for (int s = tid; s < S; s += NT) gx[s] = W.lam[s];
+        if (tid == 0) A.status[inst] = status;
+        __syncthreads();
+    }
+}
+
+int adjoint_occupancy(int *per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, reinterpret_cast<const void *>(&adjoint_kernel), NT,
+                                                        0) == hipSuccess
+               ? 0
+               : -1;
+}
+
+hipError_t launch_adjoint(const AdjArgs &args, int slots, hipStream_t stream)
+{
+    hipLaunchKernelGGL(adjoint_kernel, dim3(slots), dim3(NT), 0, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace lassie

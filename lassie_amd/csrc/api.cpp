@@ -9,6 +9,8 @@
 #include <vector>
 
 #include "../../include/lassie.h"
+#include "adjoint.h"
+#include "batch.h"
 #include "solver.h"
 
 namespace {
@@ -45,6 +47,12 @@ struct lassie_model {
     int device = 0;
     std::vector<int> lptr, lsp, lexp, hptr, hrx, hcoef;
     std::vector<double> k, x0, feed, atol, hk;
+    // index maps of the reverse pass: H^T by reaction, reactant entries by species
+    std::vector<int> htptr, htsp, htcoef, ltptr, ltent, lrx;
+    int *d_htptr = nullptr, *d_htsp = nullptr, *d_htcoef = nullptr, *d_ltptr = nullptr, *d_ltent = nullptr,
+        *d_lrx = nullptr;
+    double *d_k = nullptr;
+    unsigned long long version = 0;  // bumped by lassie_model_set_rates
     int *d_lptr = nullptr, *d_lsp = nullptr, *d_lexp = nullptr, *d_hptr = nullptr, *d_hrx = nullptr;
     double *d_hk = nullptr, *d_x0 = nullptr, *d_feed = nullptr, *d_atol = nullptr;
     lassie::HRec *d_recs = nullptr;
@@ -67,7 +75,8 @@ static void free_model(lassie_model *m)
 {
     void *ptrs[] = {m->d_lptr, m->d_lsp, m->d_lexp, m->d_hptr, m->d_hrx, m->d_hk, m->d_x0, m->d_feed,
                     m->d_atol, m->d_vec, m->d_M, m->d_piv, m->d_perm, m->d_moved, m->d_rows, m->d_rowcnt, m->d_ctl, m->d_gs,
-                    m->d_stats, m->d_dstats, m->d_tv, m->d_out, m->d_prog, m->d_cs, m->d_recs};
+                    m->d_stats, m->d_dstats, m->d_tv, m->d_out, m->d_prog, m->d_cs, m->d_recs,
+                    m->d_htptr, m->d_htsp, m->d_htcoef, m->d_ltptr, m->d_ltent, m->d_lrx, m->d_k};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -248,6 +257,41 @@ int lassie_model_create(const lassie_model_desc *d, lassie_model **out)
     if ((rc = dev_upload(&m->d_x0, m->x0.data(), m->x0.size()))) return bail(rc);
     if ((rc = dev_upload(&m->d_feed, m->feed.data(), m->feed.size()))) return bail(rc);
     if ((rc = dev_upload(&m->d_atol, m->atol.data(), m->atol.size()))) return bail(rc);
+    // reverse-pass maps (stable: entries keep the CSR order within a row)
+    m->htptr.assign(R + 1, 0);
+    for (int e = 0; e < nh; e++) m->htptr[m->hrx[e] + 1]++;
+    for (int r = 0; r < R; r++) m->htptr[r + 1] += m->htptr[r];
+    m->htsp.resize(nh);
+    m->htcoef.resize(nh);
+    {
+        std::vector<int> fill(m->htptr.begin(), m->htptr.end() - 1);
+        for (int sp = 0; sp < S; sp++)
+            for (int e = m->hptr[sp]; e < m->hptr[sp + 1]; e++) {
+                const int q = fill[m->hrx[e]]++;
+                m->htsp[q] = sp;
+                m->htcoef[q] = m->hcoef[e];
+            }
+    }
+    m->ltptr.assign(S + 1, 0);
+    m->lrx.resize(nl);
+    for (int r = 0; r < R; r++)
+        for (int j = m->lptr[r]; j < m->lptr[r + 1]; j++) {
+            m->lrx[j] = r;
+            m->ltptr[m->lsp[j] + 1]++;
+        }
+    for (int sp = 0; sp < S; sp++) m->ltptr[sp + 1] += m->ltptr[sp];
+    m->ltent.resize(nl);
+    {
+        std::vector<int> fill(m->ltptr.begin(), m->ltptr.end() - 1);
+        for (int j = 0; j < nl; j++) m->ltent[fill[m->lsp[j]]++] = j;
+    }
+    if ((rc = dev_upload(&m->d_k, m->k.data(), m->k.size()))) return bail(rc);
+    if ((rc = dev_upload(&m->d_htptr, m->htptr.data(), m->htptr.size()))) return bail(rc);
+    if ((rc = dev_upload(&m->d_htsp, m->htsp.data(), m->htsp.size()))) return bail(rc);
+    if ((rc = dev_upload(&m->d_htcoef, m->htcoef.data(), m->htcoef.size()))) return bail(rc);
+    if ((rc = dev_upload(&m->d_ltptr, m->ltptr.data(), m->ltptr.size()))) return bail(rc);
+    if ((rc = dev_upload(&m->d_ltent, m->ltent.data(), m->ltent.size()))) return bail(rc);
+    if ((rc = dev_upload(&m->d_lrx, m->lrx.data(), m->lrx.size()))) return bail(rc);
     hipError_t e;
     if ((e = hipMalloc((void **)&m->d_vec, sizeof(double) * 12 * (size_t)S)) != hipSuccess ||
         (e = hipMalloc((void **)&m->d_piv, sizeof(int) * (size_t)S)) != hipSuccess ||
@@ -444,12 +488,33 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Batched form: many uploaded models in one persistent launch.
-#include "batch.h"
+// Batched form: many uploaded models in one persistent launch; with a tape it
+// is also the forward pass of the differentiation form.
 
-extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
-                                  const double *t, int32_t nt, double *out, double *y_final,
-                                  lassie_stats *stats, double *elapsed_ms)
+struct lassie_tape {
+    int device = 0;
+    std::vector<lassie_model *> models;
+    std::vector<unsigned long long> versions;
+    lassie_opts opts{};
+    int nt = 0;
+    long long cap = 0;  // steps per instance
+    std::vector<long long> n_steps, max_kind, tape_off, ckpt_off, out_off;
+    long long out_tot = 0;
+    double *d_dt = nullptr, *d_ckpt = nullptr;
+    int *d_kind = nullptr, *d_sample = nullptr;
+};
+
+static void free_tape(lassie_tape *tp)
+{
+    void *ptrs[] = {tp->d_dt, tp->d_ckpt, tp->d_kind, tp->d_sample};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete tp;
+}
+
+static int batch_run(lassie_model *const *models, int32_t n_models, const lassie_opts *opts, const double *t,
+                     int32_t nt, double *out, double *y_final, lassie_stats *stats, double *elapsed_ms,
+                     lassie_tape *tape)
 {
     if (!models || n_models < 1) return fail(LASSIE_ERR_INVALID, "need at least one model");
     if (!t || nt < 1 || !out) return fail(LASSIE_ERR_INVALID, "need sampling times and an output buffer");
@@ -460,6 +525,8 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
         lassie_opts_default(&o);
     if (o.precision < LASSIE_PREC_FP64 || o.precision > LASSIE_PREC_BF16_REFINE)
         return fail(LASSIE_ERR_INVALID, "unknown precision");
+    if (tape && o.precision != LASSIE_PREC_FP64)
+        return fail(LASSIE_ERR_UNSUPPORTED, "the differentiation form runs the FP64 method");
     int smax = 1;
     for (int i = 0; i < n_models; i++) {
         if (!models[i]) return fail(LASSIE_ERR_INVALID, "NULL model in batch");
@@ -470,7 +537,8 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
     lassie_model *m0 = models[0];
     HIP_TRY(hipSetDevice(m0->device));
     std::vector<lassie::BatchInst> inst(n_models);
-    long long out_tot = 0, y_tot = 0;
+    long long out_tot = 0, y_tot = 0, ckpt_tot = 0;
+    const long long cap = tape ? tape->cap : 0;
     for (int i = 0; i < n_models; i++) {
         const lassie_model *m = models[i];
         lassie::BatchInst &b = inst[i];
@@ -488,8 +556,11 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
         b.atol = m->d_atol;
         b.out_off = out_tot;
         b.y_off = y_tot;
+        b.tape_off = (long long)i * cap;
+        b.ckpt_off = ckpt_tot;
         out_tot += (long long)nt * m->S;
         y_tot += m->S;
+        ckpt_tot += ((cap + lassie::kCkptEvery - 1) / lassie::kCkptEvery) * m->S;
     }
     // most expensive first (dense LU work ~ S^3) so the tail of the queue is short
     std::vector<int> order(n_models);
@@ -528,7 +599,11 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
         (e = hipMalloc((void **)&d_dst, sizeof(double) * 4 * n_models)) != hipSuccess ||
         (e = hipMalloc((void **)&d_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess ||
         (o.precision != LASSIE_PREC_FP64 &&
-         (e = hipMalloc((void **)&d_ws32, sizeof(float) * ws32_stride * slots)) != hipSuccess)) {
+         (e = hipMalloc((void **)&d_ws32, sizeof(float) * ws32_stride * slots)) != hipSuccess) ||
+        (tape && ((e = hipMalloc((void **)&tape->d_dt, sizeof(double) * cap * n_models)) != hipSuccess ||
+                  (e = hipMalloc((void **)&tape->d_kind, sizeof(int) * cap * n_models)) != hipSuccess ||
+                  (e = hipMalloc((void **)&tape->d_sample, sizeof(int) * (size_t)nt * n_models)) != hipSuccess ||
+                  (e = hipMalloc((void **)&tape->d_ckpt, sizeof(double) * std::max(1LL, ckpt_tot))) != hipSuccess))) {
         cleanup();
         return fail(LASSIE_ERR_NOMEM, std::string("batch workspace: ") + hipGetErrorString(e));
     }
@@ -546,6 +621,7 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
     BATCH_TRY(hipMemcpyAsync(d_tv, t, sizeof(double) * nt, hipMemcpyHostToDevice, st));
     BATCH_TRY(hipMemsetAsync(d_next, 0, sizeof(int), st));
     BATCH_TRY(hipMemsetAsync(d_wsi, 0, sizeof(int) * wsi_stride * slots, st));
+    if (tape) BATCH_TRY(hipMemsetAsync(tape->d_sample, 0, sizeof(int) * (size_t)nt * n_models, st));
     lassie::BatchArgs a{};
     a.inst = d_inst;
     a.n_inst = n_models;
@@ -576,6 +652,13 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
     a.ws32_stride = ws32_stride;
     a.wsi = d_wsi;
     a.wsi_stride = wsi_stride;
+    if (tape) {
+        a.tape_dt = tape->d_dt;
+        a.tape_kind = tape->d_kind;
+        a.tape_sample = tape->d_sample;
+        a.tape_ckpt = tape->d_ckpt;
+        a.tape_cap = cap;
+    }
     BATCH_TRY(hipEventRecord(m0->ev0, st));
     BATCH_TRY(lassie::launch_batch(a, slots, st));
     BATCH_TRY(hipEventRecord(m0->ev1, st));
@@ -589,6 +672,7 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
     float ms = 0.f;
     BATCH_TRY(hipEventElapsedTime(&ms, m0->ev0, m0->ev1));
     if (elapsed_ms) *elapsed_ms = ms;
+    bool maxit = false, full = false;
     for (int i = 0; i < n_models; i++) {
         const long long *c = &hst[(size_t)i * lassie::kBatchStats];
         if (stats) {
@@ -611,10 +695,243 @@ extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models,
             s.kernel_ms = ms;
             s.groups = slots;
         }
-        if (c[9]) rc = LASSIE_ERR_MAXITER;
+        maxit |= c[9] == 1;
+        full |= c[9] == 2;
+        if (tape) {
+            tape->n_steps[i] = c[12];
+            tape->max_kind[i] = c[13];
+            tape->tape_off[i] = inst[i].tape_off;
+            tape->ckpt_off[i] = inst[i].ckpt_off;
+            tape->out_off[i] = inst[i].out_off;
+        }
     }
     cleanup();
-    if (rc) return fail(rc, "max_iters reached before the last sample in at least one instance");
-    return LASSIE_OK;
+    if (tape) tape->out_tot = out_tot;
+    if (full) return fail(LASSIE_ERR_NOMEM, "step record full: raise max_steps");
+    if (maxit) return fail(LASSIE_ERR_MAXITER, "max_iters reached before the last sample in at least one instance");
+    return rc;
 #undef BATCH_TRY
+}
+
+extern "C" int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                                  const double *t, int32_t nt, double *out, double *y_final,
+                                  lassie_stats *stats, double *elapsed_ms)
+{
+    return batch_run(models, n_models, opts, t, nt, out, y_final, stats, elapsed_ms, nullptr);
+}
+
+extern "C" int lassie_model_set_rates(lassie_model *m, const double *k)
+{
+    if (!m || !k) return fail(LASSIE_ERR_INVALID, "model and k are required");
+    HIP_TRY(hipSetDevice(m->device));
+    m->k.assign(k, k + m->R);
+    const int nh = (int)m->hrx.size();
+    for (int e = 0; e < nh; e++) {
+        m->hk[e] = (double)m->hcoef[e] * m->k[m->hrx[e]];
+        m->recs[e].hk = m->hk[e];
+    }
+    HIP_TRY(hipMemcpy(m->d_k, m->k.data(), sizeof(double) * m->R, hipMemcpyHostToDevice));
+    if (nh) {
+        HIP_TRY(hipMemcpy(m->d_hk, m->hk.data(), sizeof(double) * nh, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(m->d_recs, m->recs.data(), sizeof(lassie::HRec) * nh, hipMemcpyHostToDevice));
+    }
+    m->version++;
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_forward_batch(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                                    const double *t, int32_t nt, int64_t max_steps, double *out,
+                                    lassie_stats *stats, double *elapsed_ms, lassie_tape **tape_out)
+{
+    if (!tape_out) return fail(LASSIE_ERR_INVALID, "tape_out is NULL");
+    *tape_out = nullptr;
+    if (max_steps < 1) return fail(LASSIE_ERR_INVALID, "max_steps must be positive");
+    if (!models || n_models < 1) return fail(LASSIE_ERR_INVALID, "need at least one model");
+    for (int i = 0; i < n_models; i++)
+        if (!models[i]) return fail(LASSIE_ERR_INVALID, "NULL model in batch");
+    lassie_tape *tp = new lassie_tape();
+    tp->device = models[0]->device;
+    tp->models.assign(models, models + n_models);
+    for (int i = 0; i < n_models; i++) tp->versions.push_back(models[i]->version);
+    if (opts)
+        tp->opts = *opts;
+    else
+        lassie_opts_default(&tp->opts);
+    tp->nt = nt;
+    tp->cap = max_steps;
+    tp->n_steps.assign(n_models, 0);
+    tp->max_kind.assign(n_models, 0);
+    tp->tape_off.assign(n_models, 0);
+    tp->ckpt_off.assign(n_models, 0);
+    tp->out_off.assign(n_models, 0);
+    const int rc = batch_run(models, n_models, opts, t, nt, out, nullptr, stats, elapsed_ms, tp);
+    if (rc != LASSIE_OK) {
+        const std::string keep = g_err;
+        free_tape(tp);
+        g_err = keep;
+        return rc;
+    }
+    *tape_out = tp;
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_tape_info(const lassie_tape *tp, int32_t model, int64_t *n_steps, int32_t *max_newton)
+{
+    if (!tp || model < 0 || model >= (int)tp->models.size()) return fail(LASSIE_ERR_INVALID, "bad tape or model index");
+    if (n_steps) *n_steps = tp->n_steps[model];
+    if (max_newton) *max_newton = (int32_t)tp->max_kind[model];
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_tape_read(const lassie_tape *tp, int32_t model, double *dt, int32_t *kind, int32_t *sample_step)
+{
+    if (!tp || model < 0 || model >= (int)tp->models.size()) return fail(LASSIE_ERR_INVALID, "bad tape or model index");
+    HIP_TRY(hipSetDevice(tp->device));
+    const long long n = tp->n_steps[model];
+    if (dt && n) HIP_TRY(hipMemcpy(dt, tp->d_dt + tp->tape_off[model], sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (kind && n) HIP_TRY(hipMemcpy(kind, tp->d_kind + tp->tape_off[model], sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (sample_step)
+        HIP_TRY(hipMemcpy(sample_step, tp->d_sample + (size_t)model * tp->nt, sizeof(int) * tp->nt,
+                          hipMemcpyDeviceToHost));
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_tape_destroy(lassie_tape *tp)
+{
+    if (tp) free_tape(tp);
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_adjoint_batch(lassie_tape *tp, const double *grad_out, double *grad_log_k, double *grad_x0,
+                                    double *elapsed_ms)
+{
+    if (!tp || !grad_out || !grad_log_k) return fail(LASSIE_ERR_INVALID, "tape, grad_out and grad_log_k are required");
+    const int n = (int)tp->models.size();
+    for (int i = 0; i < n; i++)
+        if (tp->models[i]->version != tp->versions[i])
+            return fail(LASSIE_ERR_INVALID, "a model's rates changed after the forward pass");
+    lassie_model *m0 = tp->models[0];
+    HIP_TRY(hipSetDevice(tp->device));
+    int smax = 1, rmax = 1, itmax = 0;
+    long long gk_tot = 0, gx_tot = 0;
+    std::vector<lassie::AdjInst> inst(n);
+    for (int i = 0; i < n; i++) {
+        const lassie_model *m = tp->models[i];
+        lassie::AdjInst &b = inst[i];
+        b.S = m->S;
+        b.R = m->R;
+        b.lptr = m->d_lptr;
+        b.lsp = m->d_lsp;
+        b.lexp = m->d_lexp;
+        b.hptr = m->d_hptr;
+        b.hrx = m->d_hrx;
+        b.recs = m->d_recs;
+        b.hk = m->d_hk;
+        b.x0 = m->d_x0;
+        b.feed = m->d_feed;
+        b.atol = m->d_atol;
+        b.k = m->d_k;
+        b.htptr = m->d_htptr;
+        b.htsp = m->d_htsp;
+        b.htcoef = m->d_htcoef;
+        b.ltptr = m->d_ltptr;
+        b.ltent = m->d_ltent;
+        b.lrx = m->d_lrx;
+        b.tape_off = tp->tape_off[i];
+        b.ckpt_off = tp->ckpt_off[i];
+        b.n_steps = tp->n_steps[i];
+        b.g_off = tp->out_off[i];
+        b.gk_off = gk_tot;
+        b.gx_off = gx_tot;
+        gk_tot += m->R;
+        gx_tot += m->S;
+        smax = std::max(smax, m->S);
+        rmax = std::max(rmax, m->R);
+        itmax = std::max(itmax, (int)tp->max_kind[i]);
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].S > inst[b].S; });
+    int cus = 0, per_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, tp->device));
+    if (lassie::adjoint_occupancy(&per_cu) != 0 || per_cu < 1) return fail(LASSIE_ERR_HIP, "adjoint occupancy query failed");
+    const int slots = std::max(1, std::min(n, cus * per_cu));
+    const int Pmax = (smax + lassie::kBatchNB - 1) / lassie::kBatchNB;
+    const size_t ws_stride = (size_t)(lassie::kCkptEvery + 1) * smax + (size_t)smax * smax +
+                             2 * (size_t)(itmax + 1) * smax + 22 * (size_t)smax + (size_t)rmax;
+    const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
+
+    lassie::AdjInst *d_inst = nullptr;
+    int *d_order = nullptr, *d_next = nullptr, *d_wsi = nullptr, *d_status = nullptr;
+    double *d_gout = nullptr, *d_gk = nullptr, *d_gx = nullptr, *d_ws = nullptr;
+    auto cleanup = [&]() {
+        void *ptrs[] = {d_inst, d_order, d_next, d_wsi, d_status, d_gout, d_gk, d_gx, d_ws};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void **)&d_inst, sizeof(lassie::AdjInst) * n)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_order, sizeof(int) * n)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_next, sizeof(int))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_status, sizeof(int) * n)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_wsi, sizeof(int) * wsi_stride * slots)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_gout, sizeof(double) * tp->out_tot)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_gk, sizeof(double) * gk_tot)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_gx, sizeof(double) * gx_tot)) != hipSuccess ||
+        (e = hipMalloc((void **)&d_ws, sizeof(double) * ws_stride * slots)) != hipSuccess) {
+        cleanup();
+        return fail(LASSIE_ERR_NOMEM, std::string("adjoint workspace: ") + hipGetErrorString(e));
+    }
+    hipStream_t st = m0->stream;
+#define ADJ_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            cleanup();                                                                           \
+            return fail(LASSIE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+        }                                                                                        \
+    } while (0)
+    ADJ_TRY(hipMemcpyAsync(d_inst, inst.data(), sizeof(lassie::AdjInst) * n, hipMemcpyHostToDevice, st));
+    ADJ_TRY(hipMemcpyAsync(d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    ADJ_TRY(hipMemcpyAsync(d_gout, grad_out, sizeof(double) * tp->out_tot, hipMemcpyHostToDevice, st));
+    ADJ_TRY(hipMemsetAsync(d_next, 0, sizeof(int), st));
+    ADJ_TRY(hipMemsetAsync(d_wsi, 0, sizeof(int) * wsi_stride * slots, st));
+    lassie::AdjArgs a{};
+    a.inst = d_inst;
+    a.n_inst = n;
+    a.order = d_order;
+    a.next = d_next;
+    a.nt = tp->nt;
+    a.tape_dt = tp->d_dt;
+    a.tape_kind = tp->d_kind;
+    a.tape_sample = tp->d_sample;
+    a.tape_ckpt = tp->d_ckpt;
+    a.grad_out = d_gout;
+    a.grad_logk = d_gk;
+    a.grad_x0 = d_gx;
+    a.rtol = tp->opts.rtol;
+    a.s_max = smax;
+    a.r_max = rmax;
+    a.it_max = itmax;
+    a.ws = d_ws;
+    a.ws_stride = ws_stride;
+    a.wsi = d_wsi;
+    a.wsi_stride = wsi_stride;
+    a.status = d_status;
+    ADJ_TRY(hipEventRecord(m0->ev0, st));
+    ADJ_TRY(lassie::launch_adjoint(a, slots, st));
+    ADJ_TRY(hipEventRecord(m0->ev1, st));
+    std::vector<int> status(n);
+    ADJ_TRY(hipMemcpyAsync(grad_log_k, d_gk, sizeof(double) * gk_tot, hipMemcpyDeviceToHost, st));
+    if (grad_x0) ADJ_TRY(hipMemcpyAsync(grad_x0, d_gx, sizeof(double) * gx_tot, hipMemcpyDeviceToHost, st));
+    ADJ_TRY(hipMemcpyAsync(status.data(), d_status, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    ADJ_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    ADJ_TRY(hipEventElapsedTime(&ms, m0->ev0, m0->ev1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    cleanup();
+    for (int i = 0; i < n; i++)
+        if (status[i]) return fail(LASSIE_ERR_INVALID, "an LU that succeeded in the forward pass failed in the replay");
+    return LASSIE_OK;
+#undef ADJ_TRY
 }

@@ -16,9 +16,11 @@ struct BatchInst {
     const HRec *recs;
     const double *hk, *x0, *feed, *atol;
     long long out_off, y_off;
+    long long tape_off, ckpt_off;  // step record of the differentiation form
 };
 
-constexpr int kBatchStats = 12;  // counters per instance
+constexpr int kBatchStats = 14;  // counters per instance
+constexpr int kCkptEvery = 32;   // steps between checkpointed states
 
 struct BatchArgs {
     const BatchInst *inst;
@@ -43,6 +45,15 @@ struct BatchArgs {
     size_t ws32_stride;
     int *wsi;      // per slot: pivots, row moves, panel row lists, flags
     size_t wsi_stride;
+    // step record (null: none): per instance, from tape_off, the size and kind
+    // of every state-changing step (kind -1 accepted RKF45, -2 BDF1 with a
+    // failed LU, else the BDF1 step's Newton iterations); the state before
+    // every kCkptEvery-th step (from ckpt_off); the steps taken at each sample
+    double *tape_dt;
+    int *tape_kind;
+    int *tape_sample;  // nt per instance
+    double *tape_ckpt;
+    long long tape_cap;
 };
 
 hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream);

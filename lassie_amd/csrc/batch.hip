@@ -14,131 +14,9 @@
 #include "batch.h"
 #include "device_common.h"
 #include "wg_common.h"
+#include "wg_lu.h"
 
 namespace lassie {
-
-__device__ __forceinline__ float to_bf16_rne(float x)
-{
-    // round to the nearest BF16 value (ties to even), kept in an FP32 register
-    unsigned int u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return __uint_as_float(u & 0xffff0000u);
-}
-
-// Workgroup-local blocked right-looking LU of a row-major n x n matrix with
-// partial pivoting (panel_factor semantics).  Row moves go to every column,
-// so on return M holds L\U in final row order.  Returns 0 or k+1 of the
-// first zero pivot.  BF16 = round the trailing-update operands to BF16.
-template <typename T, bool BF16>
-__device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
-                        int *info, Shared &sh)
-{
-    constexpr int NBC = kBatchNB;
-    constexpr int RPT = kBatchMaxS / NT;
-    const int P = (n + NBC - 1) / NBC;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    if (tid == 0) st_rlx(info, 0);
-    __syncthreads();
-    for (int p = 0; p < P; p++) {
-        const int r0 = p * NBC;
-        const int nbp = min(NBC, n - r0);
-        panel_factor<RPT, NBC, T, true>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n, rowcnt + p, info,
-                                        sh);
-        __syncthreads();
-        const int inf = ld_rlx(info);
-        if (inf) return inf;
-        for (int e = tid; e < NBC * NBC; e += NT) {
-            const int kk = e / NBC, c = e % NBC;
-            if (kk < nbp && c < kk) sh.L11[kk][c] = (double)M[(size_t)(r0 + kk) * n + r0 + c];
-        }
-        __syncthreads();
-        // row moves in every other column; U block row by forward substitution
-        int nonfinite = 0;
-        for (int c = tid; c < n; c += NT) {
-            if (c >= r0 && c < r0 + nbp) continue;
-            T vals[NBC], old[NBC];
-#pragma unroll
-            for (int kk = 0; kk < NBC; kk++) {
-                vals[kk] = (kk < nbp) ? M[(size_t)perm_src[r0 + kk] * n + c] : T(0);
-                old[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + c] : T(0);
-            }
-#pragma unroll
-            for (int kk = 0; kk < NBC; kk++) {
-                if (kk < nbp) {
-                    const int d = moved_to[r0 + kk];
-                    if (d >= r0 + nbp) M[(size_t)d * n + c] = old[kk];
-                }
-            }
-            if (c >= r0 + nbp) {
-#pragma unroll
-                for (int i = 1; i < NBC; i++) {
-                    if (i < nbp) {
-#pragma unroll
-                        for (int k = 0; k < i; k++) vals[i] = fma(-(T)sh.L11[i][k], vals[k], vals[i]);
-                    }
-                }
-#pragma unroll
-                for (int kk = 0; kk < NBC; kk++) nonfinite |= (kk < nbp) && !isfinite((double)vals[kk]);
-            }
-#pragma unroll
-            for (int kk = 0; kk < NBC; kk++)
-                if (kk < nbp) M[(size_t)(r0 + kk) * n + c] = vals[kk];
-        }
-        const bool all_rows = __syncthreads_or(nonfinite) != 0;
-        // trailing update over rows with nonzero multipliers: lanes take 64
-        // consecutive columns, waves take rows, two rows in flight per wave
-        const int c0 = r0 + nbp;
-        if (c0 < n) {
-            const int cnt = all_rows ? n - c0 : rowcnt[p];
-            const int *rl = rows + (size_t)p * n;
-            for (int cb = c0; cb < n; cb += 64) {
-                const int c = cb + lane;
-                const bool okc = c < n;
-                T u[NBC];
-#pragma unroll
-                for (int kk = 0; kk < NBC; kk++) {
-                    T uv = (okc && kk < nbp) ? M[(size_t)(r0 + kk) * n + c] : T(0);
-                    if (BF16) uv = (T)to_bf16_rne((float)uv);
-                    u[kk] = uv;
-                }
-                for (int idx = wave; idx < cnt; idx += 2 * NWAVES) {
-                    const int idx1 = idx + NWAVES;
-                    const bool h1 = idx1 < cnt;
-                    const int i0 = all_rows ? c0 + idx : rl[idx];
-                    const int i1 = h1 ? (all_rows ? c0 + idx1 : rl[idx1]) : i0;
-                    T l0[NBC], l1[NBC];
-#pragma unroll
-                    for (int kk = 0; kk < NBC; kk++) {
-                        T a = (kk < nbp) ? M[(size_t)i0 * n + r0 + kk] : T(0);
-                        T b = (kk < nbp && h1) ? M[(size_t)i1 * n + r0 + kk] : T(0);
-                        if (BF16) {
-                            a = (T)to_bf16_rne((float)a);
-                            b = (T)to_bf16_rne((float)b);
-                        }
-                        l0[kk] = a;
-                        l1[kk] = b;
-                    }
-                    T a0 = okc ? M[(size_t)i0 * n + c] : T(0);
-                    T a1 = (okc && h1) ? M[(size_t)i1 * n + c] : T(0);
-#pragma unroll
-                    for (int kk = 0; kk < NBC; kk++) {
-                        if (kk < nbp) {
-                            a0 = fma(-l0[kk], u[kk], a0);
-                            a1 = fma(-l1[kk], u[kk], a1);
-                        }
-                    }
-                    if (okc) M[(size_t)i0 * n + c] = a0;
-                    if (okc && h1) M[(size_t)i1 * n + c] = a1;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    return 0;
-}
 
 // Mixed-precision Newton correction: solve M d = rhs with the FP32 factors and
 // FP64 residuals r = rhs - M64 x, until every component meets the
@@ -240,6 +118,24 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
         int rc = 0;
         for (int s = tid; s < S; s += NT) v.y1[s] = I.x0[s];
         __syncthreads();
+        // step record: called by every thread before the state changes
+        const bool rec = A.tape_dt != nullptr;
+        long long nstep = 0, max_kind = 0;
+        auto record = [&](double dstep, int kind) -> bool {
+            if (!rec) return true;
+            if (nstep >= A.tape_cap) return false;
+            if (nstep % kCkptEvery == 0) {
+                double *ck = A.tape_ckpt + I.ckpt_off + (nstep / kCkptEvery) * S;
+                for (int s = tid; s < S; s += NT) ck[s] = v.y1[s];
+            }
+            if (tid == 0) {
+                A.tape_dt[I.tape_off + nstep] = dstep;
+                A.tape_kind[I.tape_off + nstep] = kind;
+            }
+            max_kind = max(max_kind, (long long)kind);
+            nstep++;
+            return true;
+        };
 
         while (true) {
             if (A.max_iters > 0 && st_loop >= A.max_iters) {
@@ -249,6 +145,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             st_loop++;
             if (fabs(t1 - t) < 1e-5) {
                 for (int j = tid; j < S; j += NT) out[(size_t)i_sample * S + j] = v.y1[j];
+                if (rec && tid == 0) A.tape_sample[(size_t)inst * A.nt + i_sample] = (int)nstep;
                 i_sample++;
                 if (i_sample == A.nt) break;
                 t1 = A.tv[i_sample];
@@ -259,6 +156,11 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             if (nok == S) {
                 nrej = 0;
                 st_acc++;
+                if (!record(dt, -1)) {
+                    rc = 2;
+                    break;
+                }
+                __syncthreads();
                 for (int s = tid; s < S; s += NT) v.y1[s] = v.w1[s];
                 __syncthreads();
                 const double d_temp = dt * mind;
@@ -284,6 +186,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             for (int s = tid; s < S; s += NT) v.yv[s] = v.y1[s];
             __syncthreads();
             bool luFail = false;
+            int itN = 0;
             if (A.newton_iter > 0) {
                 const size_t nn = (size_t)S * S;
                 for (size_t e = tid; e < nn; e += NT) M[e] = 0.0;
@@ -305,7 +208,6 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                 }
                 if (fp64) inf = wg_lu_rm<double, false>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
                 st_lu++;
-                int itN = 0;
                 double errN = 1000;
                 while (itN < A.newton_iter && errN > A.newton_tol) {
                     for (int s = tid; s < S; s += NT) {
@@ -368,6 +270,11 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                 }
             }
             __syncthreads();
+            if (!record(h, luFail ? -2 : itN)) {
+                rc = 2;
+                break;
+            }
+            __syncthreads();
             for (int s = tid; s < S; s += NT) v.y1[s] = v.yv[s];
             __syncthreads();
             t = t + h;
@@ -387,6 +294,8 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             o[9] = rc;
             o[10] = st_ref;
             o[11] = st_fb;
+            o[12] = nstep;
+            o[13] = max_kind;
             double *d = A.dstats + (size_t)inst * 4;
             d[0] = t;
             d[1] = dt;

@@ -150,6 +150,24 @@ def uneven_batch(n: int, seed_base: int = 1000, sizes=(16, 32, 64, 128, 256, 512
             for i, p in enumerate(pick)]
 
 
+def parameterisations(S: int, R: int, B: int, seed: int = 0, k_range=(1e-4, 1e4),
+                      hidden_seed: int = 7) -> tuple[list[Model], np.ndarray]:
+    """B parameterisations of one stiff network (structure and x0 from
+    ``seed``): parameterisation b has its own log-uniform rates (seed
+    ``seed + 1 + b``).  Also returns the hidden rates k* = k * U[0.5, 2]
+    (seed ``hidden_seed``), [B, R], that generate a target trajectory."""
+    base = stiff(S, R, seed, k_range)
+    lk0, lk1 = np.log10(k_range[0]), np.log10(k_range[1])
+    models, hidden = [], []
+    hrng = np.random.default_rng(hidden_seed)
+    for b in range(B):
+        k = 10.0 ** np.random.default_rng(seed + 1 + b).uniform(lk0, lk1, size=R)
+        models.append(model_from_dense(base.L, base.Rm, k, base.x0,
+                                       meta=dict(base.meta, parameterisation=b)))
+        hidden.append(k * hrng.uniform(0.5, 2.0, size=R))
+    return models, np.array(hidden)
+
+
 def t_vector(t0: float, t1: float, n: int) -> np.ndarray:
     """Equispaced sampling times, as the LASSIE GUI writes them
     (``gui/lassie-gui.py:341-344``)."""

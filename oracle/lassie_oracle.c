@@ -54,36 +54,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "lassie_oracle.h"
+
 #define NORM_LANES 256
 
-typedef struct {
-    int32_t S, R;
-    const int32_t *lptr, *lsp, *lexp;   /* reactants: reaction-major CSR */
-    const int32_t *hptr, *hrx, *hcoef;  /* net stoichiometry: species-major CSR */
-    const double *k, *x0, *feed, *atol;
-} orc_model;
-
-typedef struct {
-    double rkf_step;      /* Simulation.cuh:281-288, default 1e-3 */
-    double be_step;       /* Simulation.cuh:272-279, default 1e-2 */
-    double newton_tol;    /* Simulation.cuh:299-306, default 1e-6 */
-    double stiffness_tol; /* Simulation.cuh:308-315, default 1e-6 */
-    double rtol;          /* extension; 0 = reference behaviour */
-    int32_t newton_iter;  /* Simulation.cuh:290-297, default 10000 */
-    int32_t bdf_regrow;   /* extension: 1 = BDF step doubles back towards be_step */
-    int32_t max_rkf_retries; /* extension: >0 = this many rejections in a row force BDF */
-    int32_t pad_;
-    int64_t max_iters;    /* safety cap on loop iterations, 0 = none */
-} orc_opts;
-
-typedef struct {
-    int64_t loop_iters, rkf_attempts, rkf_accepts, bdf_steps, newton_iters;
-    int64_t lu_failures, nan_repairs, samples;
-    double t_final, dt_final, newdt_final;
-} orc_stats;
-
 /* myPow, kernels.cuh:52-63 */
-static double mypow(double base, int ex)
+double orc_mypow(double base, int ex)
 {
     double p = 1.0;
     while (ex > 0) { p = p * base; ex--; }
@@ -102,8 +78,8 @@ void orc_rhs(const orc_model *m, const double *x, double *f)
             if (a1 > a0) {
                 double temp = 0.0;
                 for (int j = a0; j < a1; j++) {
-                    if (j == a0) temp = temp + sk * mypow(x[m->lsp[j]], m->lexp[j]);
-                    else temp = temp * mypow(x[m->lsp[j]], m->lexp[j]);
+                    if (j == a0) temp = temp + sk * orc_mypow(x[m->lsp[j]], m->lexp[j]);
+                    else temp = temp * orc_mypow(x[m->lsp[j]], m->lexp[j]);
                 }
                 acc = acc + temp;
             } else {
@@ -135,7 +111,7 @@ void orc_jac(const orc_model *m, const double *x, double *J)
                 if (ex == 1) {
                     temp = temp + (double)c * m->k[r];
                 } else if (ex > 1) {
-                    temp = temp + (((double)c * m->k[r]) * (double)ex) * mypow(x[g], ex - 1);
+                    temp = temp + (((double)c * m->k[r]) * (double)ex) * orc_mypow(x[g], ex - 1);
                 } else {
                     continue;
                 }
@@ -246,13 +222,9 @@ double orc_min_first(int n, const double *d)
     return m;
 }
 
-typedef struct {
-    double *k1, *k2, *k3, *k4, *k5, *k6, *u, *w1, *w2, *delta;
-} rkf_ws;
-
 /* One RKF45 attempt (Simulation.cuh:692-740): fills w1, w2, delta; returns the
  * number of species that pass calculate_delta's test. */
-static int rkf_attempt(const orc_model *m, double rtol, double dt, const double *y1, rkf_ws *w)
+int orc_rkf_ws_attempt(const orc_model *m, double rtol, double dt, const double *y1, orc_rkf_ws *w)
 {
     const int S = m->S;
     const double *fd = m->feed;
@@ -327,15 +299,15 @@ int orc_rkf_attempt(const orc_model *m, double rtol, double dt, const double *y1
 {
     const int S = m->S;
     double *buf = (double *)malloc(sizeof(double) * (size_t)S * 7);
-    rkf_ws w = {buf, buf + S, buf + 2 * S, buf + 3 * S, buf + 4 * S, buf + 5 * S, buf + 6 * S,
+    orc_rkf_ws w = {buf, buf + S, buf + 2 * S, buf + 3 * S, buf + 4 * S, buf + 5 * S, buf + 6 * S,
                 w1, w2, delta};
-    int nok = rkf_attempt(m, rtol, dt, y1, &w);
+    int nok = orc_rkf_ws_attempt(m, rtol, dt, y1, &w);
     free(buf);
     return nok;
 }
 
 /* aggiornaNaN, kernels.cuh:512-521 */
-static int repair(int S, double *yv, const double *y1)
+int orc_repair(int S, double *yv, const double *y1)
 {
     int n = 0;
     for (int s = 0; s < S; s++)
@@ -343,7 +315,7 @@ static int repair(int S, double *yv, const double *y1)
     return n;
 }
 
-static int count_bad(int S, const double *yv)
+int orc_count_bad(int S, const double *yv)
 {
     int n = 0;
     for (int s = 0; s < S; s++)
@@ -351,16 +323,11 @@ static int count_bad(int S, const double *yv)
     return n;
 }
 
-typedef struct {
-    double *J, *x, *yv, *yv1;
-    int32_t *piv;
-} bdf_ws;
-
 /* One BDF1 step solved by modified Newton (Simulation.cuh:780-919).  y1 is
  * replaced by the new state; returns the step size taken.  *newdt is the
  * persistent BDF step (newDtBDF). */
 static double bdf_step(const orc_model *m, const orc_opts *o, double t, double t1, double *newdt,
-                       double *y1, bdf_ws *w, orc_stats *st)
+                       double *y1, orc_bdf_ws *w, orc_stats *st, int32_t *kind)
 {
     const int S = m->S;
     double dt = *newdt;
@@ -379,7 +346,7 @@ static double bdf_step(const orc_model *m, const orc_opts *o, double t, double t
             info = orc_lu(S, w->J, w->piv);
         }
         if (info != 0) {
-            repair(S, w->yv, y1);
+            orc_repair(S, w->yv, y1);
             *newdt = dt * 0.1;
             luFail = 1;
             st->lu_failures++;
@@ -396,9 +363,10 @@ static double bdf_step(const orc_model *m, const orc_opts *o, double t, double t
         itN = itN + 1;
     }
     st->newton_iters += itN;
+    *kind = luFail ? ORC_STEP_LUFAIL : itN;
     if (!luFail) {
-        if (count_bad(S, w->yv) > 0) {
-            repair(S, w->yv, y1);
+        if (orc_count_bad(S, w->yv) > 0) {
+            orc_repair(S, w->yv, y1);
             *newdt = dt * 0.1;
             st->nan_repairs++;
         } else if (o->bdf_regrow) {
@@ -419,15 +387,33 @@ int orc_solve(const orc_model *m, const orc_opts *o, const double *tv, int nt, c
               int ncs, double *out, double *y_final, double *progress, int64_t progress_cap,
               orc_stats *st)
 {
+    return orc_solve_rec(m, o, tv, nt, cs, ncs, out, y_final, progress, progress_cap, st, NULL);
+}
+
+/* Appends one step to the tape (stops recording on overflow). */
+static void tape_push(orc_tape *tape, double dt, int32_t kind)
+{
+    if (!tape) return;
+    if (tape->n >= tape->cap) { tape->overflow = 1; return; }
+    tape->dt[tape->n] = dt;
+    tape->kind[tape->n] = kind;
+    tape->n++;
+}
+
+/* orc_solve that also records the steps it takes (tape may be NULL). */
+int orc_solve_rec(const orc_model *m, const orc_opts *o, const double *tv, int nt, const int32_t *cs,
+                  int ncs, double *out, double *y_final, double *progress, int64_t progress_cap,
+                  orc_stats *st, orc_tape *tape)
+{
     const int S = m->S;
     orc_stats local;
     if (!st) st = &local;
     memset(st, 0, sizeof(*st));
     double *y1 = (double *)malloc(sizeof(double) * S);
     double *buf = (double *)malloc(sizeof(double) * (size_t)S * 14);
-    rkf_ws rw = {buf, buf + S, buf + 2 * S, buf + 3 * S, buf + 4 * S, buf + 5 * S, buf + 6 * S,
+    orc_rkf_ws rw = {buf, buf + S, buf + 2 * S, buf + 3 * S, buf + 4 * S, buf + 5 * S, buf + 6 * S,
                  buf + 7 * S, buf + 8 * S, buf + 9 * S};
-    bdf_ws bw;
+    orc_bdf_ws bw;
     bw.J = (double *)malloc(sizeof(double) * (size_t)S * S);
     bw.x = buf + 10 * S;
     bw.yv = buf + 11 * S;
@@ -443,24 +429,27 @@ int orc_solve(const orc_model *m, const orc_opts *o, const double *tv, int nt, c
     const double t_max = tv[nt - 1];
     int rc = 0;
     int64_t nrej = 0;
+    if (tape) { tape->n = 0; tape->overflow = 0; }
     while (i < nt) {
         if (o->max_iters > 0 && st->loop_iters >= o->max_iters) { rc = 1; break; }
         if (progress && st->loop_iters < progress_cap) progress[st->loop_iters] = t / t_max;
         st->loop_iters++;
         if (fabs(t1 - t) < 1e-5) {
             for (int j = 0; j < ncs; j++) out[(size_t)i * ncs + j] = y1[cs[j]];
+            if (tape) tape->sample_step[i] = (int32_t)tape->n;
             i++;
             st->samples = i;
             if (i == nt) break;
             t1 = tv[i];
         }
         st->rkf_attempts++;
-        int nok = rkf_attempt(m, o->rtol, dt, y1, &rw);
+        int nok = orc_rkf_ws_attempt(m, o->rtol, dt, y1, &rw);
         double mind = orc_min_first(S, rw.delta);
         if (nok == S) {
             nrej = 0;
             st->rkf_accepts++;
             memcpy(y1, rw.w1, sizeof(double) * S);
+            tape_push(tape, dt, ORC_STEP_RKF);
             double d_temp = dt * mind;
             if ((t + d_temp) > t1) dt = fabs(t1 - t);
             else dt = d_temp;
@@ -473,7 +462,9 @@ int orc_solve(const orc_model *m, const orc_opts *o, const double *tv, int nt, c
                 || (o->max_rkf_retries > 0 && nrej >= o->max_rkf_retries)) {
                 nrej = 0;
                 st->bdf_steps++;
-                dt = bdf_step(m, o, t, t1, &newdt, y1, &bw, st);
+                int32_t kind = 0;
+                dt = bdf_step(m, o, t, t1, &newdt, y1, &bw, st, &kind);
+                tape_push(tape, dt, kind);
                 t = t + dt;
             }
         }

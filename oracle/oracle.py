@@ -44,6 +44,15 @@ class OrcStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class OrcTape(C.Structure):
+    _fields_ = [("cap", C.c_int64), ("n", C.c_int64), ("dt", _f64p), ("kind", _i32p),
+                ("sample_step", _i32p), ("overflow", C.c_int32), ("pad_", C.c_int32)]
+
+
+STEP_RKF = -1
+STEP_LUFAIL = -2
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -75,6 +84,16 @@ def lib():
         L.orc_solve.restype = C.c_int
         L.orc_set_lu_variant.argtypes = [C.c_int]
         L.orc_set_lu_variant.restype = None
+        L.orc_solve_rec.argtypes = L.orc_solve.argtypes + [C.POINTER(OrcTape)]
+        L.orc_solve_rec.restype = C.c_int
+        L.orc_replay.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcOpts), C.POINTER(OrcTape), C.c_int, _f64p]
+        L.orc_replay.restype = C.c_int
+        L.orc_tangent.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcOpts), C.POINTER(OrcTape), C.c_int, _f64p,
+                                  _f64p, _f64p]
+        L.orc_tangent.restype = C.c_int
+        L.orc_adjoint.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcOpts), C.POINTER(OrcTape), C.c_int, _f64p,
+                                  _f64p, _f64p]
+        L.orc_adjoint.restype = C.c_int
         _lib = L
     return _lib
 
@@ -191,3 +210,88 @@ def set_lu_variant(two_roundings: bool):
     arithmetic) and separately rounded multiply/subtract (another valid FP64
     LU, for measuring the FP64 noise floor of trajectories)."""
     lib().orc_set_lu_variant(1 if two_roundings else 0)
+
+
+class Tape:
+    """Steps of one recorded solve: kind[n] (STEP_RKF, STEP_LUFAIL or the
+    number of Newton iterations of a BDF1 step), dt[n], and sample_step[i]
+    (steps taken when sample i was stored)."""
+
+    def __init__(self, cap, nt):
+        self.dt = np.zeros(cap)
+        self.kind = np.zeros(cap, np.int32)
+        self.sample_step = np.zeros(nt, np.int32)
+        self.s = OrcTape(cap, 0, _p(self.dt, _f64p), _p(self.kind, _i32p), _p(self.sample_step, _i32p), 0, 0)
+        self.opts = None
+
+    @property
+    def n(self):
+        return self.s.n
+
+    def steps(self):
+        return self.kind[:self.n].copy(), self.dt[:self.n].copy()
+
+
+def solve_tape(m, t, cap=1 << 20, **kw):
+    """Full trajectory of every species plus the recorded step sequence:
+    returns (out[nt, S], stats, rc, tape)."""
+    b = _Bound(m)
+    t = np.ascontiguousarray(t, np.float64)
+    cs = np.arange(m.S, dtype=np.int32)
+    out = np.full((t.size, m.S), np.nan)
+    o = opts(**kw)
+    st = OrcStats()
+    tape = Tape(cap, t.size)
+    rc = lib().orc_solve_rec(C.byref(b.s), C.byref(o), _p(t, _f64p), t.size, _p(cs, _i32p), m.S,
+                             _p(out, _f64p), None, None, 0, C.byref(st), C.byref(tape.s))
+    if tape.s.overflow:
+        raise RuntimeError("tape capacity exceeded")
+    tape.opts = kw
+    return out, st.as_dict(), rc, tape
+
+
+def _with_params(m, k=None, x0=None):
+    import dataclasses
+    return dataclasses.replace(m, k=m.k if k is None else np.asarray(k, np.float64),
+                               x0=m.x0 if x0 is None else np.asarray(x0, np.float64))
+
+
+def replay(m, tape, nt, k=None, x0=None):
+    """States at the samples, the recorded steps re-run with rate constants k
+    and initial state x0 (defaults: the model's)."""
+    mm = _with_params(m, k, x0)
+    b = _Bound(mm)
+    o = opts(**tape.opts)
+    out = np.full((nt, m.S), np.nan)
+    rc = lib().orc_replay(C.byref(b.s), C.byref(o), C.byref(tape.s), nt, _p(out, _f64p))
+    if rc != 0:
+        raise RuntimeError(f"replay failed ({rc})")
+    return out
+
+
+def tangent(m, tape, nt, dlogk, dx0=None):
+    """Forward sensitivity of the sampled states along (dlogk, dx0)."""
+    b = _Bound(m)
+    o = opts(**tape.opts)
+    dlogk = np.ascontiguousarray(dlogk, np.float64)
+    dx0 = np.zeros(m.S) if dx0 is None else np.ascontiguousarray(dx0, np.float64)
+    dout = np.zeros((nt, m.S))
+    rc = lib().orc_tangent(C.byref(b.s), C.byref(o), C.byref(tape.s), nt, _p(dlogk, _f64p), _p(dx0, _f64p),
+                           _p(dout, _f64p))
+    if rc != 0:
+        raise RuntimeError(f"tangent failed ({rc})")
+    return dout
+
+
+def adjoint(m, tape, gout):
+    """Reverse pass: (dL/dlog k [R], dL/dx0 [S]) for gout[i, s] = dL/d(state s
+    at sample i)."""
+    b = _Bound(m)
+    o = opts(**tape.opts)
+    gout = np.ascontiguousarray(gout, np.float64)
+    gk, gx = np.zeros(m.R), np.zeros(m.S)
+    rc = lib().orc_adjoint(C.byref(b.s), C.byref(o), C.byref(tape.s), gout.shape[0], _p(gout, _f64p),
+                           _p(gk, _f64p), _p(gx, _f64p))
+    if rc != 0:
+        raise RuntimeError(f"adjoint failed ({rc})")
+    return gk, gx
