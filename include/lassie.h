@@ -107,7 +107,8 @@ int lassie_model_destroy(lassie_model *model);
 
 /* Integrates from t = 0 (Simulation.cuh:632-636) and stores the state of
  * species cs[j] at every sampling time t[i] into out[i*ncs + j]
- * (saveDinamic, kernels.cuh:524-535).  y_final[S] (may be NULL) receives the
+ * (saveDinamic, kernels.cuh:524-535); rows of samples the loop did not reach
+ * (LASSIE_ERR_MAXITER) are left unchanged.  y_final[S] (may be NULL) receives the
  * state when the loop ends; progress (may be NULL, capacity progress_cap)
  * receives t/t_max at the top of every loop iteration (ProgressSimulation,
  * Simulation.cuh:660-672).  Reference: Simulation<T>::run, Simulation.cuh:230-981. */

@@ -149,7 +149,7 @@ __device__ void lu_solve_t_wg(const double *__restrict__ M, int n, const int *pi
         if (tid < 64) {
             double bi = lane < nb ? b[k0 + lane] : 0.0;
             for (int kk = 0; kk < nb; kk++) {
-                const double w = __shfl(bi, kk) / M[(size_t)(k0 + kk) * n + k0 + kk];
+                const double w = readlane_d(bi, kk) / M[(size_t)(k0 + kk) * n + k0 + kk];
                 if (lane == kk) bi = w;
                 if (lane > kk && lane < nb) bi -= M[(size_t)(k0 + kk) * n + k0 + lane] * w;
             }
@@ -168,7 +168,7 @@ __device__ void lu_solve_t_wg(const double *__restrict__ M, int n, const int *pi
         if (tid < 64) {
             double bi = lane < nb ? b[k0 + lane] : 0.0;
             for (int kk = nb - 1; kk >= 0; kk--) {
-                const double u = __shfl(bi, kk);
+                const double u = readlane_d(bi, kk);
                 if (lane < kk) bi -= M[(size_t)(k0 + kk) * n + k0 + lane] * u;
             }
             if (lane < nb) b[k0 + lane] = bi;

@@ -450,10 +450,12 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     HIP_TRY(hipMemcpyAsync(dst, m->d_dstats, sizeof(double) * 7, hipMemcpyDeviceToHost, m->stream));
     lassie::Ctl ctl;
     HIP_TRY(hipMemcpyAsync(&ctl, m->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, m->stream));
-    if (ncs) HIP_TRY(hipMemcpyAsync(out, m->d_out, sizeof(double) * (size_t)nt * ncs, hipMemcpyDeviceToHost, m->stream));
     if (y_final) HIP_TRY(hipMemcpyAsync(y_final, a.gvec, sizeof(double) * S, hipMemcpyDeviceToHost, m->stream));
     int64_t nprog = 0;
     HIP_TRY(hipStreamSynchronize(m->stream));
+    // only the stored samples (rows past them are left untouched when max_iters stops the loop)
+    const size_t rows = (size_t)std::min<long long>(std::max<long long>(st[7], 0), nt);
+    if (ncs && rows) HIP_TRY(hipMemcpy(out, m->d_out, sizeof(double) * rows * ncs, hipMemcpyDeviceToHost));
     if (ctl.abort) return fail(LASSIE_ERR_DEVICE, "device-side wait exceeded its bound");
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));

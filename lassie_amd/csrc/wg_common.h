@@ -9,9 +9,10 @@ namespace lassie {
 struct Shared {
     double red[NT];
     int ired[NT];
-    double cand_row[2][NWAVES][16];
+    double cand_row[2][NWAVES][16];  // pivot candidates, one per wave
     double cand_v[2][NWAVES];
     int cand_p[2][NWAVES];
+    int qcount[16][NWAVES];  // panel row-list counts per (row slot, wave)
     double U12[16][16];
     double L11[16][16];
     double oldP[16][16];
@@ -241,6 +242,62 @@ __device__ void jac_row(const DevModel &m, const double *__restrict__ x, double 
 }
 
 // ---------------------------------------------------------------------------
+// Cross-lane helpers without LDS traffic.  DPP moves within 16-lane rows
+// (quad_perm, row_half_mirror, row_mirror) and readlane from a uniform lane.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Pivot choice of one wave: the largest v, then the smallest position p among
+// equal v (a total order, so the result does not depend on the reduction
+// tree).  Values first (DPP within 16-lane rows, then the four rows by
+// readlane), then the smallest position among the lanes holding that value.
+// Branch-free; the results are wave-uniform.
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ void wave_argmax(double v, int p, double &wv, int &wp)
+{
+    double m = v;
+    m = fmax(m, dpp_mov_d<0xB1>(m));   // quad_perm [1,0,3,2]
+    m = fmax(m, dpp_mov_d<0x4E>(m));   // quad_perm [2,3,0,1]
+    m = fmax(m, dpp_mov_d<0x141>(m));  // row_half_mirror
+    m = fmax(m, dpp_mov_d<0x140>(m));  // row_mirror
+    m = fmax(fmax(readlane_d(m, 0), readlane_d(m, 16)), fmax(readlane_d(m, 32), readlane_d(m, 48)));
+    int q = (v == m) ? p : 0x7fffffff;
+    q = min(q, dpp_mov_i<0xB1>(q));
+    q = min(q, dpp_mov_i<0x4E>(q));
+    q = min(q, dpp_mov_i<0x141>(q));
+    q = min(q, dpp_mov_i<0x140>(q));
+    q = min(min(__builtin_amdgcn_readlane(q, 0), __builtin_amdgcn_readlane(q, 16)),
+            min(__builtin_amdgcn_readlane(q, 32), __builtin_amdgcn_readlane(q, 48)));
+    wv = m;
+    wp = q;
+}
+
+// v of a uniform lane, to every lane (float overload; the double one is above)
+__device__ __forceinline__ float readlane_d(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// ---------------------------------------------------------------------------
 // Panel factorization with partial pivoting of columns [r0, r0+nbp), rows
 // [r0, n), held in registers: RPT rows per thread, NBC columns.  Pivot: the
 // largest |a| among remaining rows, first (lowest position) on ties, a NaN at
@@ -267,7 +324,6 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
     T a[RPT][NBC];
     int pos[RPT];
     bool act[RPT];
-    if (tid == 0) sh.rowcnt = 0;
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int r = r0 + tid + NT * q;
@@ -278,113 +334,126 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
     }
     int buf = 0;
     bool stop = false;
+    // Columns run without control flow around them (a branch per column made
+    // the compiler copy the whole register panel at every merge).  Padding
+    // columns (j >= nbp) and columns after a zero pivot compute values that
+    // are never used: they change no position, pivot or stored entry.
 #pragma unroll
     for (int j = 0; j < NBC; j++) {
-        if (j < nbp && !stop) {
-            const int k = r0 + j;
-            double bv = -2.0;
-            int bp = 0x7fffffff;
-            int bq = -1;
+        const bool live = j < nbp && !stop;
+        const int k = r0 + j;
+        double bv = -2.0;
+        int bp = 0x7fffffff;
+        int bq = -1;
 #pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                if (act[q]) {
-                    const double xv = (double)a[q][j];
-                    double v = fabs(xv);
-                    if (isnan(xv)) v = (pos[q] == k) ? INFINITY : -1.0;
-                    if (v > bv || (v == bv && pos[q] < bp)) {
-                        bv = v;
-                        bp = pos[q];
-                        bq = q;
-                    }
+        for (int q = 0; q < RPT; q++) {
+            const double xv = (double)a[q][j];
+            const double v = isnan(xv) ? ((pos[q] == k) ? INFINITY : -1.0) : fabs(xv);
+            const bool take = act[q] && (v > bv || (v == bv && pos[q] < bp));
+            bv = take ? v : bv;
+            bp = take ? pos[q] : bp;
+            bq = take ? q : bq;
+        }
+        double wv;
+        int wp;
+        wave_argmax(bv, bp, wv, wp);
+        if (lane == 0) {
+            sh.cand_v[buf][wave] = wv;
+            sh.cand_p[buf][wave] = wp;
+        }
+        if (bq >= 0 && bp == wp) {
+#pragma unroll
+            for (int q = 0; q < RPT; q++)
+                if (q == bq) {
+#pragma unroll
+                    for (int c = 0; c < NBC; c++) sh.cand_row[buf][wave][c] = (double)a[q][c];
                 }
-            }
-            const int mybp = bp;
+        }
+        __syncthreads();
+        int w = 0;
+        bv = sh.cand_v[buf][0];
+        bp = sh.cand_p[buf][0];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(bv, off);
-                const int op = __shfl_xor(bp, off);
-                if (ov > bv || (ov == bv && op < bp)) {
-                    bv = ov;
-                    bp = op;
-                }
-            }
-            if (lane == 0) {
-                sh.cand_v[buf][wave] = bv;
-                sh.cand_p[buf][wave] = bp;
-            }
-            if (bq >= 0 && mybp == bp) {
+        for (int ww = 1; ww < NWAVES; ww++) {
+            const double ov = sh.cand_v[buf][ww];
+            const int op = sh.cand_p[buf][ww];
+            const bool take = ov > bv || (ov == bv && op < bp);
+            bv = take ? ov : bv;
+            bp = take ? op : bp;
+            w = take ? ww : w;
+        }
+        const double *prow = sh.cand_row[buf][w];
+        const T ukk = (T)prow[j];
+        if (live && ukk == T(0)) {
+            if (tid == 0) st_rlx(info, k + 1);
+            stop = true;
+        }
+        const bool go = live && ukk != T(0);
+        if (go && tid == 0) piv[k] = bp;
 #pragma unroll
-                for (int q = 0; q < RPT; q++)
-                    if (q == bq) {
+        for (int q = 0; q < RPT; q++) {
+            const int old = pos[q];
+            const bool isp = go && act[q] && old == bp;
+            const bool isk = go && act[q] && old == k;
+            pos[q] = isp ? k : (isk ? bp : old);
+            act[q] = act[q] && !isp;
+        }
+        T pr[NBC];
 #pragma unroll
-                        for (int c = 0; c < NBC; c++) sh.cand_row[buf][wave][c] = (double)a[q][c];
-                    }
-            }
-            __syncthreads();
-            int w = 0;
-            bv = sh.cand_v[buf][0];
-            bp = sh.cand_p[buf][0];
+        for (int c = 0; c < NBC; c++) pr[c] = (c > j) ? (T)prow[c] : T(0);
+        // rows already used as pivots keep their values
 #pragma unroll
-            for (int ww = 1; ww < NWAVES; ww++) {
-                const double ov = sh.cand_v[buf][ww];
-                const int op = sh.cand_p[buf][ww];
-                if (ov > bv || (ov == bv && op < bp)) {
-                    bv = ov;
-                    bp = op;
-                    w = ww;
-                }
-            }
-            const double *prow = sh.cand_row[buf][w];
-            const T ukk = (T)prow[j];
-            if (ukk == T(0)) {
-                if (tid == 0) st_rlx(info, k + 1);
-                stop = true;
-            } else {
-                if (tid == 0) piv[k] = bp;
+        for (int q = 0; q < RPT; q++) {
+            if (go && act[q]) {
+                const T l = a[q][j] / ukk;
+                a[q][j] = l;
 #pragma unroll
-                for (int q = 0; q < RPT; q++) {
-                    const int old = pos[q];
-                    if (act[q] && old == bp) {
-                        act[q] = false;
-                        pos[q] = k;
-                    } else if (act[q] && old == k) {
-                        pos[q] = bp;
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < RPT; q++) {
-                    if (act[q]) {
-                        const T l = a[q][j] / ukk;
-                        a[q][j] = l;
-#pragma unroll
-                        for (int c = 0; c < NBC; c++)
-                            if (c > j) a[q][c] = fma(-l, (T)prow[c], a[q][c]);
-                    }
-                }
-                buf ^= 1;
+                for (int c = 0; c < NBC; c++)
+                    if (c > j) a[q][c] = fma(-l, pr[c], a[q][c]);
             }
         }
+        buf ^= 1;
     }
     if (stop) return;
+    // rows below the panel with a nonzero multiplier, listed by (q, wave, lane)
+    bool nzq[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int orig = r0 + tid + NT * q;
+        bool nz = false;
+        if (orig < n && act[q]) {
+#pragma unroll
+            for (int c = 0; c < NBC; c++) nz = nz || (c < nbp && a[q][c] != T(0));
+        }
+        nzq[q] = nz;
+        const unsigned long long m = __ballot(nz);
+        if (lane == 0) sh.qcount[q][wave] = __popcll(m);
+    }
+    __syncthreads();
+    int total = 0, base[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++)
+#pragma unroll
+        for (int w = 0; w < NWAVES; w++) {
+            if (w == wave) base[q] = total;
+            total += sh.qcount[q][w];
+        }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int orig = r0 + tid + NT * q;
+        const unsigned long long m = __ballot(nzq[q]);
+        const int off = __builtin_amdgcn_mbcnt_hi((int)(m >> 32), __builtin_amdgcn_mbcnt_lo((int)m, 0));
         if (orig < n) {
 #pragma unroll
             for (int c = 0; c < NBC; c++)
                 if (c < nbp) at(pos[q], r0 + c) = a[q][c];
             if (!act[q]) perm_src[pos[q]] = orig;
             if (orig < r0 + nbp) moved_to[orig] = pos[q];
-            if (act[q]) {
-                bool nz = false;
-#pragma unroll
-                for (int c = 0; c < NBC; c++) nz = nz || (c < nbp && a[q][c] != T(0));
-                if (nz) rows[atomicAdd(&sh.rowcnt, 1)] = pos[q];
-            }
+            if (nzq[q]) rows[base[q] + off] = pos[q];
         }
     }
+    if (tid == 0) *rowcnt = total;
     __syncthreads();
-    if (tid == 0) *rowcnt = sh.rowcnt;
 }
 
 // getrs with one right-hand side (Simulation.cuh:1316), one group, b in LDS.
@@ -439,7 +508,7 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
                 for (int kk = 0; kk < 16; kk++) {
                     const int k = sb * 16 + kk;
                     if (k < nb) {
-                        const T bk = __shfl(bi, k);
+                        const T bk = readlane_d(bi, k);
                         if (lane > k && lane < nb) bi = fma(-lsub[kk], bk, bi);
                     }
                 }
@@ -495,7 +564,7 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
                     const int k = sb * 16 + kk;
                     if (k < nb) {
                         if (lane == k) bi = bi / usub[kk];
-                        const T xk = __shfl(bi, k);
+                        const T xk = readlane_d(bi, k);
                         if (lane < k) bi = fma(-usub[kk], xk, bi);
                     }
                 }
