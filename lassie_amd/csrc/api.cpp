@@ -60,6 +60,10 @@ struct lassie_model {
     double *d_vec = nullptr;  // work vectors, 12 x S
     double *d_M = nullptr;    // dense S x S Newton matrix (column-major)
     int *d_piv = nullptr, *d_perm = nullptr, *d_moved = nullptr, *d_rows = nullptr, *d_rowcnt = nullptr;
+    int *d_pfull = nullptr;             // final row order of the factors
+    double *d_tri = nullptr;            // distributed triangular solves: 3 x S
+    unsigned int *d_tflag = nullptr;    // their per-block stamps
+    size_t tflag_words = 0;
     lassie::Ctl *d_ctl = nullptr;
     lassie::GridSync *d_gs = nullptr;
     long long *d_stats = nullptr;
@@ -76,7 +80,8 @@ static void free_model(lassie_model *m)
     void *ptrs[] = {m->d_lptr, m->d_lsp, m->d_lexp, m->d_hptr, m->d_hrx, m->d_hk, m->d_x0, m->d_feed,
                     m->d_atol, m->d_vec, m->d_M, m->d_piv, m->d_perm, m->d_moved, m->d_rows, m->d_rowcnt, m->d_ctl, m->d_gs,
                     m->d_stats, m->d_dstats, m->d_tv, m->d_out, m->d_prog, m->d_cs, m->d_recs,
-                    m->d_htptr, m->d_htsp, m->d_htcoef, m->d_ltptr, m->d_ltent, m->d_lrx, m->d_k};
+                    m->d_htptr, m->d_htsp, m->d_htcoef, m->d_ltptr, m->d_ltent, m->d_lrx, m->d_k,
+                    m->d_pfull, m->d_tri, m->d_tflag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -319,6 +324,14 @@ int lassie_model_create(const lassie_model_desc *d, lassie_model **out)
             fail(LASSIE_ERR_NOMEM, std::string("panel row lists: ") + hipGetErrorString(e));
             return bail(LASSIE_ERR_NOMEM);
         }
+        const int nblk = (S + lassie::kTriBlock - 1) / lassie::kTriBlock;
+        m->tflag_words = 2 * (size_t)nblk * lassie::kFlagStride;
+        if ((e = hipMalloc((void **)&m->d_pfull, sizeof(int) * (size_t)S)) != hipSuccess ||
+            (e = hipMalloc((void **)&m->d_tri, sizeof(double) * 3 * (size_t)S)) != hipSuccess ||
+            (e = hipMalloc((void **)&m->d_tflag, sizeof(unsigned int) * m->tflag_words)) != hipSuccess) {
+            fail(LASSIE_ERR_NOMEM, std::string("triangular-solve buffers: ") + hipGetErrorString(e));
+            return bail(LASSIE_ERR_NOMEM);
+        }
         if ((e = hipMalloc((void **)&m->d_M, sizeof(double) * (size_t)S * S)) != hipSuccess) {
             fail(LASSIE_ERR_NOMEM, std::string("dense Newton matrix: ") + hipGetErrorString(e));
             return bail(LASSIE_ERR_NOMEM);
@@ -379,6 +392,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     HIP_TRY(hipMemsetAsync(m->d_ctl, 0, sizeof(lassie::Ctl), m->stream));
     HIP_TRY(hipMemsetAsync(m->d_gs, 0, sizeof(lassie::GridSync), m->stream));
     HIP_TRY(hipMemsetAsync(m->d_stats, 0, sizeof(long long) * 16, m->stream));
+    if (m->d_tflag) HIP_TRY(hipMemsetAsync(m->d_tflag, 0, sizeof(unsigned int) * m->tflag_words, m->stream));
 
     const int S = m->S;
     const int nb = lassie::dense_panel_width(S);
@@ -434,6 +448,9 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.moved_to = m->d_moved;
     a.rows = m->d_rows;
     a.rowcnt = m->d_rowcnt;
+    a.perm_full = m->d_pfull;
+    a.tri = m->d_tri;
+    a.tflag = m->d_tflag;
     a.ctl = m->d_ctl;
     a.gs = m->d_gs;
     a.stats_out = m->d_stats;

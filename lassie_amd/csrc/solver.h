@@ -23,7 +23,13 @@ struct Ctl {
     int pad2[15];
     double h[2];
     double pad4[6];
+    unsigned int ngo;    // Newton iteration stamp published by group 0
+    int ncont;           // 1: solve this iteration, 0: the Newton loop ends
+    int pad5[14];
 };
+
+constexpr int kTriBlock = 64;     // rows per block of the distributed triangular solves
+constexpr int kFlagStride = 16;   // ints between per-block flags (one 64-byte line each)
 
 struct SolveArgs {
     DevModel m;
@@ -44,6 +50,9 @@ struct SolveArgs {
     double *M;
     int *piv, *perm_src, *moved_to;
     int *rows, *rowcnt;    // per panel: positions with nonzero multipliers
+    int *perm_full;        // final row order of the factors: row i of L U is row perm_full[i] of M
+    double *tri;           // 3 x S: permuted right-hand side, forward result, solution
+    unsigned int *tflag;   // 2 x blocks x kFlagStride: forward / backward block stamps
     Ctl *ctl;
     GridSync *gs;
     long long *stats_out;  // 10 counters

@@ -196,6 +196,157 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
 }
 
 // ---------------------------------------------------------------------------
+// Distributed triangular solves of the Newton correction (getrs,
+// Simulation.cuh:1316).  Row block j (kTriBlock rows) belongs to wave 0 of
+// group j % G.  Blocks are chained by per-block stamps: the writer stores its
+// block, then publishes the stamp after an agent-scope fence; a reader waits
+// for the stamp, fences, then loads the block.  Each element's fma chain runs
+// in the order of lu_solve_wg and the oracle: forward k ascending (earlier
+// blocks first, then the diagonal block), backward k descending.
+
+// Wait (every lane of the calling wave) until *flag == want; bounded.
+__device__ __forceinline__ bool wait_stamp(const unsigned int *flag, unsigned int want, int *abort_word)
+{
+    const unsigned long long t0 = wall_clock64();
+    unsigned int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+        if ((++spins & 255) == 0) {
+            if (wall_clock64() - t0 > kWaitLimit) {
+                st_rlx(abort_word, 1);
+                return false;
+            }
+            if (ld_rlx(abort_word)) return false;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// Every lane releases its own stores, then lane 0 publishes the stamp.
+__device__ __forceinline__ void publish_stamp(unsigned int *flag, unsigned int v)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(flag, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// L y = b for block j (wave 0 of its owner): b = tri[0..S), y -> tri[S..2S).
+__device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp)
+{
+    constexpr int TB = kTriBlock;
+    const int n = A.m.S;
+    const int lane = threadIdx.x & 63;
+    const int R0 = j * TB;
+    const int nb = min(TB, n - R0);
+    const int i = R0 + lane;
+    const bool row = lane < nb;
+    const double *M = A.M;
+    const double *bp = A.tri;
+    double *yb = A.tri + n;
+    unsigned int *fwd = A.tflag;
+    // the diagonal block's L entries first: the chain below must not wait on memory
+    double d[TB];
+#pragma unroll
+    for (int k = 0; k < TB; k++) d[k] = (k < lane && row) ? M[(size_t)(R0 + k) * n + i] : 0.0;
+    double acc = row ? bp[i] : 0.0;
+    for (int B = 0; B < j; B++) {
+        const int c0 = B * TB;
+        double l[16], ln[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) l[kk] = row ? M[(size_t)(c0 + kk) * n + i] : 0.0;
+        if (!wait_stamp(fwd + (size_t)B * kFlagStride, stamp, &A.ctl->abort)) return false;
+        const double yB = yb[c0 + lane];
+#pragma unroll
+        for (int sb = 0; sb < TB / 16; sb++) {
+            if (sb + 1 < TB / 16) {
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) ln[kk] = row ? M[(size_t)(c0 + (sb + 1) * 16 + kk) * n + i] : 0.0;
+            }
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) acc = fma(-l[kk], readlane_d(yB, sb * 16 + kk), acc);
+            if (sb + 1 < TB / 16) {
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) l[kk] = ln[kk];
+            }
+        }
+    }
+    // diagonal block, unit lower triangle
+#pragma unroll
+    for (int k = 0; k < TB; k++) {
+        if (k < nb) {
+            const double yk = readlane_d(acc, k);
+            if (lane > k && row) acc = fma(-d[k], yk, acc);
+        }
+    }
+    if (row) yb[i] = acc;
+    publish_stamp(fwd + (size_t)j * kFlagStride, stamp);
+    return true;
+}
+
+// U x = y for block j: y = tri[S..2S), x -> tri[2S..3S).
+__device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned int stamp)
+{
+    constexpr int TB = kTriBlock;
+    const int n = A.m.S;
+    const int lane = threadIdx.x & 63;
+    const int R0 = j * TB;
+    const int nb = min(TB, n - R0);
+    const int i = R0 + lane;
+    const bool row = lane < nb;
+    const double *M = A.M;
+    const double *yb = A.tri + n;
+    double *xb = A.tri + 2 * (size_t)n;
+    unsigned int *bwd = A.tflag + (size_t)nblk * kFlagStride;
+    // the diagonal block's U entries (with the diagonal) first
+    double d[TB];
+#pragma unroll
+    for (int k = 0; k < TB; k++) d[k] = (k >= lane && k < nb && row) ? M[(size_t)(R0 + k) * n + i] : 1.0;
+    double acc = row ? yb[i] : 0.0;
+    for (int B = nblk - 1; B > j; B--) {
+        const int c0 = B * TB;
+        const int nbB = min(TB, n - c0);
+        double u[16], un[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            const int k = TB - 16 + kk;
+            u[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : 0.0;
+        }
+        if (!wait_stamp(bwd + (size_t)B * kFlagStride, stamp, &A.ctl->abort)) return false;
+        const double xB = (lane < nbB) ? xb[c0 + lane] : 0.0;
+#pragma unroll
+        for (int sb = TB / 16 - 1; sb >= 0; sb--) {
+            if (sb > 0) {
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                    const int k = (sb - 1) * 16 + kk;
+                    un[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int kk = 15; kk >= 0; kk--) {
+                const int k = sb * 16 + kk;
+                if (k < nbB) acc = fma(-u[kk], readlane_d(xB, k), acc);
+            }
+            if (sb > 0) {
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) u[kk] = un[kk];
+            }
+        }
+    }
+    // diagonal block, upper triangle with the diagonal
+#pragma unroll
+    for (int k = TB - 1; k >= 0; k--) {
+        if (k < nb) {
+            if (lane == k) acc = acc / d[k];
+            const double xk = readlane_d(acc, k);
+            if (lane < k) acc = fma(-d[k], xk, acc);
+        }
+    }
+    if (row) xb[i] = acc;
+    publish_stamp(bwd + (size_t)j * kFlagStride, stamp);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 template <int RPT, int NBC>
 __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
 {
@@ -271,6 +422,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     __syncthreads();
 
     unsigned int parity = 0;
+    unsigned int tri_stamp = 0;  // Newton iterations seen by this group (identical in every participant)
     while (true) {
         parity ^= 1;
         // ---- control step (group 0) -----------------------------------------
@@ -389,48 +541,116 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 tmark = now;
             }
         }
-        // ---- Newton loop, group 0 --------------------------------------------
-        if (wg == 0) {
-            st_lu += (mode == MODE_BDF) ? 1 : 0;
-            bool luFail = false;
-            if (mode == MODE_BDF) {
-                int itN = 0;
-                double errN = 1000;
-                while (itN < A.newton_iter && errN > A.newton_tol) {
-                    if (itN > 0) {
-                        for (int s = tid; s < S; s += NT) {
-                            const double f = rhs_species(A.m, v.recs, v.hptr, v.yv, s);
-                            v.x[s] = -(v.yv[s] - v.y1[s] - h * f);
+        // ---- Newton loop -------------------------------------------------------
+        // Group 0 forms each residual and updates the iterate; the triangular
+        // solves are spread over the owners of the row blocks (tri_*_block).
+        bool luFail = false;
+        if (mode == MODE_BDF) {
+            const int nblk = (n + kTriBlock - 1) / kTriBlock;
+            if (wg == 0) {
+                st_lu++;
+                if (!singular) {
+                    // final row order of the factors: the panels' row moves composed
+                    int *pf = A.perm_full;
+                    for (int s = tid; s < n; s += NT) pf[s] = s;
+                    __syncthreads();
+                    const int P = (n + NBC - 1) / NBC;
+                    for (int p = 0; p < P; p++) {
+                        const int r0 = p * NBC;
+                        const int nbp = min(NBC, n - r0);
+                        int v1 = 0, v2 = 0, dst = -1;
+                        if (tid < nbp) {
+                            v1 = pf[A.perm_src[r0 + tid]];
+                            v2 = pf[r0 + tid];
+                            dst = A.moved_to[r0 + tid];
+                        }
+                        __syncthreads();
+                        if (tid < nbp) {
+                            pf[r0 + tid] = v1;
+                            if (dst >= r0 + nbp) pf[dst] = v2;
                         }
                         __syncthreads();
                     }
-                    if (singular) {
-                        for (int s = tid; s < S; s += NT)
-                            if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
-                        __syncthreads();
-                        newdt = h * 0.1;
-                        luFail = true;
-                        st_lufail++;
-                        break;
-                    }
-                    lu_solve_wg<NBC, double, false>(A.M, n, A.perm_src, A.moved_to, v.x);
-                    double part = 0.0;
-                    for (int s = tid; s < S; s += NT) {
-                        const double yv1 = v.yv[s];
-                        double yv;
-                        if (A.m.feed[s] == 0)
-                            yv = yv1 + v.x[s];
-                        else
-                            yv = A.m.feed[s];
-                        v.yv[s] = yv;
-                        const double d = yv - yv1;
-                        part = part + (double)(float)(d * d);
-                    }
-                    errN = sqrt(wg_tree_sum(part, sh));
-                    itN = itN + 1;
                 }
-                st_newton += itN;
             }
+            if (wg < nblk) {
+                __shared__ int s_cont;
+                int itN = 0;
+                double errN = 1000;
+                while (true) {
+                    tri_stamp++;
+                    if (wg == 0) {
+                        int cont = (itN < A.newton_iter && errN > A.newton_tol) ? 1 : 0;
+                        if (cont && itN > 0) {
+                            for (int s = tid; s < S; s += NT) {
+                                const double f = rhs_species(A.m, v.recs, v.hptr, v.yv, s);
+                                v.x[s] = -(v.yv[s] - v.y1[s] - h * f);
+                            }
+                            __syncthreads();
+                        }
+                        if (cont && singular) {
+                            for (int s = tid; s < S; s += NT)
+                                if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
+                            __syncthreads();
+                            newdt = h * 0.1;
+                            luFail = true;
+                            st_lufail++;
+                            cont = 0;
+                        }
+                        if (cont)
+                            for (int s = tid; s < S; s += NT) A.tri[s] = v.x[A.perm_full[s]];
+                        __syncthreads();
+                        if (tid == 0) {
+                            __threadfence();
+                            st_rlx(&ctl->ncont, cont);
+                            __threadfence();
+                            __hip_atomic_store(&ctl->ngo, tri_stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            s_cont = cont;
+                        }
+                    } else if (tid == 0) {
+                        int cont = -1;
+                        if (wait_stamp(&ctl->ngo, tri_stamp, &ctl->abort)) cont = ld_rlx(&ctl->ncont);
+                        s_cont = cont;
+                    }
+                    __syncthreads();
+                    const int cont = s_cont;
+                    __syncthreads();
+                    if (cont < 0) return;
+                    if (cont == 0) break;
+                    __shared__ int s_ok;
+                    if (tid < 64) {
+                        bool ok = true;
+                        for (int j = wg; j < nblk && ok; j += G) ok = tri_forward_block(A, j, tri_stamp);
+                        int jl = wg;
+                        while (jl + G < nblk) jl += G;
+                        for (int j = jl; j >= 0 && ok; j -= G) ok = tri_backward_block(A, j, nblk, tri_stamp);
+                        if (tid == 0) s_ok = ok ? 1 : 0;
+                    }
+                    __syncthreads();
+                    if (!s_ok) return;
+                    if (wg == 0) {
+                        __threadfence();
+                        const double *xs = A.tri + 2 * (size_t)n;
+                        double part = 0.0;
+                        for (int s = tid; s < S; s += NT) {
+                            const double yv1 = v.yv[s];
+                            double yv;
+                            if (A.m.feed[s] == 0)
+                                yv = yv1 + xs[s];
+                            else
+                                yv = A.m.feed[s];
+                            v.yv[s] = yv;
+                            const double d = yv - yv1;
+                            part = part + (double)(float)(d * d);
+                        }
+                        errN = sqrt(wg_tree_sum(part, sh));
+                        itN = itN + 1;
+                    }
+                }
+                if (wg == 0) st_newton += itN;
+            }
+        }
+        if (wg == 0) {
             if (!luFail) {
                 int nbad = 0;
                 for (int s = tid; s < S; s += NT) nbad += bad_value(v.yv[s]) ? 1 : 0;
