@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
         W.seg = p;
         p += (size_t)(kCkptEvery + 1) * smax;
         W.M = p;
-        p += (size_t)smax * smax;
+        p += (size_t)smax * smax + kLuSink;
         W.V = p;
         p += (size_t)(A.it_max + 1) * smax;
         W.X = p;

@@ -591,8 +591,8 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     if (lassie::batch_occupancy(&per_cu) != 0 || per_cu < 1) return fail(LASSIE_ERR_HIP, "batch occupancy query failed");
     const int slots = std::max(1, std::min(n_models, cus * per_cu));
     const int Pmax = (smax + lassie::kBatchNB - 1) / lassie::kBatchNB;
-    const size_t ws64_stride = 12 * (size_t)smax + (size_t)smax * smax;
-    const size_t ws32_stride = (size_t)smax * smax;
+    const size_t ws64_stride = 12 * (size_t)smax + (size_t)smax * smax + lassie::kLuSink;
+    const size_t ws32_stride = (size_t)smax * smax + lassie::kLuSink;
     const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
 
     lassie::BatchInst *d_inst = nullptr;
@@ -713,6 +713,7 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
             s.newdt_final = hd[4 * i + 2];
             s.kernel_ms = ms;
             s.groups = slots;
+            for (int q = 0; q < 4; q++) s.phase_ms[q] = (double)c[14 + q] * 1e-5;
         }
         maxit |= c[9] == 1;
         full |= c[9] == 2;
@@ -876,7 +877,7 @@ extern "C" int lassie_adjoint_batch(lassie_tape *tp, const double *grad_out, dou
     if (lassie::adjoint_occupancy(&per_cu) != 0 || per_cu < 1) return fail(LASSIE_ERR_HIP, "adjoint occupancy query failed");
     const int slots = std::max(1, std::min(n, cus * per_cu));
     const int Pmax = (smax + lassie::kBatchNB - 1) / lassie::kBatchNB;
-    const size_t ws_stride = (size_t)(lassie::kCkptEvery + 1) * smax + (size_t)smax * smax +
+    const size_t ws_stride = (size_t)(lassie::kCkptEvery + 1) * smax + (size_t)smax * smax + lassie::kLuSink +
                              2 * (size_t)(itmax + 1) * smax + 22 * (size_t)smax + (size_t)rmax;
     const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
 

@@ -8,6 +8,7 @@ namespace lassie {
 
 constexpr int kBatchMaxS = 512;  // largest instance of the batched form
 constexpr int kBatchNB = 16;     // panel width of the workgroup-local LU
+constexpr int kLuSink = 64;      // elements after each workgroup matrix that take discarded stores
 
 // One instance: device pointers of an uploaded model and where its outputs go.
 struct BatchInst {
@@ -19,7 +20,7 @@ struct BatchInst {
     long long tape_off, ckpt_off;  // step record of the differentiation form
 };
 
-constexpr int kBatchStats = 14;  // counters per instance
+constexpr int kBatchStats = 18;  // counters per instance (last 4: phase times, 100 MHz ticks)
 constexpr int kCkptEvery = 32;   // steps between checkpointed states
 
 struct BatchArgs {

@@ -113,6 +113,8 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
         (void)t_max;
         int i_sample = 0;
         long long nrej = 0;
+        // device time by phase (100 MHz ticks): RKF45 attempts, Jacobian, LU, Newton
+        unsigned long long ph_rkf = 0, ph_jac = 0, ph_lu = 0, ph_nw = 0, tph = 0;
         long long st_loop = 0, st_att = 0, st_acc = 0, st_bdf = 0, st_newton = 0, st_lufail = 0, st_rep = 0,
                   st_lu = 0, st_ref = 0, st_fb = 0;
         int rc = 0;
@@ -152,7 +154,9 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             }
             st_att++;
             double mind;
+            tph = wall_clock64();
             const int nok = rkf_attempt_wg(m, A.rtol, v, dt, &mind, sh);
+            ph_rkf += wall_clock64() - tph;
             if (nok == S) {
                 nrej = 0;
                 st_acc++;
@@ -188,11 +192,17 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             bool luFail = false;
             int itN = 0;
             if (A.newton_iter > 0) {
+                tph = wall_clock64();
                 const size_t nn = (size_t)S * S;
                 for (size_t e = tid; e < nn; e += NT) M[e] = 0.0;
                 __syncthreads();
                 for (int r = tid; r < S; r += NT) jac_row<true>(m, v.yv, M, S, r, h);
                 __syncthreads();
+                {
+                    const unsigned long long now = wall_clock64();
+                    ph_jac += now - tph;
+                    tph = now;
+                }
                 bool fp64 = A.precision == 0;
                 int inf = 0;
                 if (!fp64) {
@@ -208,6 +218,11 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                 }
                 if (fp64) inf = wg_lu_rm<double, false>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
                 st_lu++;
+                {
+                    const unsigned long long now = wall_clock64();
+                    ph_lu += now - tph;
+                    tph = now;
+                }
                 double errN = 1000;
                 while (itN < A.newton_iter && errN > A.newton_tol) {
                     for (int s = tid; s < S; s += NT) {
@@ -255,6 +270,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                     itN = itN + 1;
                 }
                 st_newton += itN;
+                ph_nw += wall_clock64() - tph;
             }
             if (!luFail) {
                 int nbad = 0;
@@ -296,6 +312,10 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             o[11] = st_fb;
             o[12] = nstep;
             o[13] = max_kind;
+            o[14] = (long long)ph_rkf;
+            o[15] = (long long)ph_jac;
+            o[16] = (long long)ph_lu;
+            o[17] = (long long)ph_nw;
             double *d = A.dstats + (size_t)inst * 4;
             d[0] = t;
             d[1] = dt;

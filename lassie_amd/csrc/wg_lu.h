@@ -16,8 +16,34 @@ __device__ __forceinline__ float to_bf16_rne(float x)
     return __uint_as_float(u & 0xffff0000u);
 }
 
+// 16x16x4 matrix-core step D = A B + C for FP64 and FP32 operands.
+template <typename T>
+struct Mfma;
+// Both take A lane l -> A[l%16][l/16] and B lane l -> B[l/16][l%16]; item i
+// of lane l holds D[row(l/16, i)][l%16], whose row order differs between the
+// two instructions (measured on gfx950).
+template <>
+struct Mfma<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc run(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lq, int i) { return lq + 4 * i; }
+};
+template <>
+struct Mfma<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc run(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lq, int i) { return 4 * lq + i; }
+};
+
 // Workgroup-local blocked right-looking LU of a row-major n x n matrix with
-// partial pivoting (panel_factor semantics).  Row moves go to every column,
+// partial pivoting (panel_factor semantics).  M must be followed by kLuSink
+// writable elements.  Row moves go to every column,
 // so on return M holds L\U in final row order.  Returns 0 or k+1 of the
 // first zero pivot.  BF16 = round the trailing-update operands to BF16.
 template <typename T, bool BF16>
@@ -46,7 +72,6 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
         }
         __syncthreads();
         // row moves in every other column; U block row by forward substitution
-        int nonfinite = 0;
         for (int c = tid; c < n; c += NT) {
             if (c >= r0 && c < r0 + nbp) continue;
             T vals[NBC], old[NBC];
@@ -70,58 +95,99 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
                         for (int k = 0; k < i; k++) vals[i] = fma(-(T)sh.L11[i][k], vals[k], vals[i]);
                     }
                 }
-#pragma unroll
-                for (int kk = 0; kk < NBC; kk++) nonfinite |= (kk < nbp) && !isfinite((double)vals[kk]);
             }
 #pragma unroll
             for (int kk = 0; kk < NBC; kk++)
                 if (kk < nbp) M[(size_t)(r0 + kk) * n + c] = vals[kk];
         }
-        const bool all_rows = __syncthreads_or(nonfinite) != 0;
-        // trailing update over rows with nonzero multipliers: lanes take 64
-        // consecutive columns, waves take rows, two rows in flight per wave
+        __syncthreads();
+        // trailing update A22 -= L21 U12 on the matrix cores: 16 x 16 output
+        // tiles, four 16x16x4 MFMAs each (k ascending; the FP64 instruction
+        // accumulates as a sequential fma chain, the oracle's order).  Every
+        // row is updated, as in the oracle.  A wave owns the column tiles
+        // ct = wave (mod 4), keeps their U12 operands in registers and walks
+        // the row tiles RT at a time, the next RT tiles' operands in flight.
         const int c0 = r0 + nbp;
         if (c0 < n) {
-            const int cnt = all_rows ? n - c0 : rowcnt[p];
-            const int *rl = rows + (size_t)p * n;
-            for (int cb = c0; cb < n; cb += 64) {
-                const int c = cb + lane;
-                const bool okc = c < n;
-                T u[NBC];
+            constexpr int RT = 4;
+            typedef typename Mfma<T>::acc Acc;
+            // global-memory view of M (the workspace lives in device memory):
+            // global loads/stores keep the memory counters precise, so the
+            // next tiles' loads stay in flight across the current MFMAs
+            typedef __attribute__((address_space(1))) T GT;
+            GT *Mg = (GT *)M;
+            const size_t last = (size_t)n * n - 1;
+            const size_t sink = (size_t)n * n;  // discarded stores (kLuSink elements after M)
+            const int nt = (n - c0 + 15) >> 4;
+            const int l16 = lane & 15, lq = lane >> 4;
+            for (int ct = wave; ct < nt; ct += NWAVES) {
+                const int col = c0 + 16 * ct + l16;
+                const bool okc = col < n;
+                T b[4];
 #pragma unroll
-                for (int kk = 0; kk < NBC; kk++) {
-                    T uv = (okc && kk < nbp) ? M[(size_t)(r0 + kk) * n + c] : T(0);
-                    if (BF16) uv = (T)to_bf16_rne((float)uv);
-                    u[kk] = uv;
+                for (int kc = 0; kc < 4; kc++) {
+                    const int k = 4 * kc + lq;
+                    const bool ok = okc && k < nbp;
+                    T bv = Mg[ok ? (size_t)(r0 + k) * n + col : 0];
+                    bv = ok ? bv : T(0);
+                    if (BF16) bv = (T)to_bf16_rne((float)bv);
+                    b[kc] = bv;
                 }
-                for (int idx = wave; idx < cnt; idx += 2 * NWAVES) {
-                    const int idx1 = idx + NWAVES;
-                    const bool h1 = idx1 < cnt;
-                    const int i0 = all_rows ? c0 + idx : rl[idx];
-                    const int i1 = h1 ? (all_rows ? c0 + idx1 : rl[idx1]) : i0;
-                    T l0[NBC], l1[NBC];
+                // branch-free loads: out-of-range lanes read a valid address and discard it
+                auto load_group = [&](int g, Acc *dd, T (*aa)[4]) {
 #pragma unroll
-                    for (int kk = 0; kk < NBC; kk++) {
-                        T a = (kk < nbp) ? M[(size_t)i0 * n + r0 + kk] : T(0);
-                        T b = (kk < nbp && h1) ? M[(size_t)i1 * n + r0 + kk] : T(0);
-                        if (BF16) {
-                            a = (T)to_bf16_rne((float)a);
-                            b = (T)to_bf16_rne((float)b);
-                        }
-                        l0[kk] = a;
-                        l1[kk] = b;
-                    }
-                    T a0 = okc ? M[(size_t)i0 * n + c] : T(0);
-                    T a1 = (okc && h1) ? M[(size_t)i1 * n + c] : T(0);
+                    for (int u = 0; u < RT; u++) {
+                        const int R = c0 + 16 * (g + u);
+                        const bool okt = g + u < nt;
 #pragma unroll
-                    for (int kk = 0; kk < NBC; kk++) {
-                        if (kk < nbp) {
-                            a0 = fma(-l0[kk], u[kk], a0);
-                            a1 = fma(-l1[kk], u[kk], a1);
+                        for (int i = 0; i < 4; i++) {
+                            const int r = R + Mfma<T>::row(lq, i);
+                            const bool ok = okt && r < n && okc;
+                            const T v = Mg[ok ? (size_t)r * n + col : last];
+                            dd[u][i] = ok ? v : T(0);
+                        }
+                        const int ra = R + l16;
+#pragma unroll
+                        for (int kc = 0; kc < 4; kc++) {
+                            const int k = 4 * kc + lq;
+                            const bool ok = okt && ra < n && k < nbp;
+                            const T v = Mg[ok ? (size_t)ra * n + r0 + k : last];
+                            // padding products are (-0) * 0 = -0, which leaves every value unchanged
+                            T av = ok ? -v : T(-0.0);
+                            if (BF16) av = (T)to_bf16_rne((float)av);
+                            aa[u][kc] = av;
                         }
                     }
-                    if (okc) M[(size_t)i0 * n + c] = a0;
-                    if (okc && h1) M[(size_t)i1 * n + c] = a1;
+                };
+                Acc d[RT], dn[RT];
+                T a[RT][4], an[RT][4];
+                load_group(0, d, a);
+                for (int g = 0; g < nt; g += RT) {
+                    const bool more = g + RT < nt;
+                    load_group(g + RT, dn, an);  // unconditional: no branch, exact memory counters
+#pragma unroll
+                    for (int u = 0; u < RT; u++) {
+#pragma unroll
+                        for (int kc = 0; kc < 4; kc++) d[u] = Mfma<T>::run(a[u][kc], b[kc], d[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < RT; u++) {
+                        const int R = c0 + 16 * (g + u);
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int r = R + Mfma<T>::row(lq, i);
+                            const bool ok = g + u < nt && r < n && okc;
+                            Mg[ok ? (size_t)r * n + col : sink] = d[u][i];
+                        }
+                    }
+                    if (more) {
+#pragma unroll
+                        for (int u = 0; u < RT; u++) {
+                            d[u] = dn[u];
+#pragma unroll
+                            for (int kc = 0; kc < 4; kc++) a[u][kc] = an[u][kc];
+                        }
+                    }
                 }
             }
         }
