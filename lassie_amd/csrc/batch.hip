@@ -38,15 +38,20 @@ __device__ bool refine_solve(const double *__restrict__ M64, const float *__rest
         lu_solve_wg<kBatchNB, float, true>(M32, n, perm_src, moved_to, b32);
         for (int i = tid; i < n; i += NT) x[i] = x[i] + (double)b32[i];
         __syncthreads();
+        // r = rhs - M64 x, one wave per row, lanes over columns (coalesced)
         int bad = 0;
-        for (int i = tid; i < n; i += NT) {
+        const int lane = tid & 63, wave = tid >> 6;
+        for (int i = wave; i < n; i += NWAVES) {
             const double *row = M64 + (size_t)i * n;
-            double acc = rhs[i], mag = fabs(rhs[i]);
-            for (int j = 0; j < n; j++) {
-                acc = fma(-row[j], x[j], acc);
-                mag = fma(fabs(row[j]), fabs(x[j]), mag);
+            double pa = 0.0, pm = 0.0;
+            for (int j = lane; j < n; j += 64) {
+                const double mv = row[j], xv = x[j];
+                pa = fma(mv, xv, pa);
+                pm = fma(fabs(mv), fabs(xv), pm);
             }
-            b32[i] = (float)acc;
+            const double acc = rhs[i] - wave_sum(pa);
+            const double mag = fabs(rhs[i]) + wave_sum(pm);
+            if (lane == 0) b32[i] = (float)acc;
             bad |= !(fabs(acc) <= tol * mag);
         }
         if (__syncthreads_or(bad) == 0) return true;

@@ -291,6 +291,17 @@ __device__ __forceinline__ void wave_argmax(double v, int p, double &wv, int &wp
     wp = q;
 }
 
+// Sum over the wave in a fixed order (DPP within 16-lane rows, then the four
+// row sums by readlane); every lane receives the result.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v = v + dpp_mov_d<0xB1>(v);
+    v = v + dpp_mov_d<0x4E>(v);
+    v = v + dpp_mov_d<0x141>(v);
+    v = v + dpp_mov_d<0x140>(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
 // v of a uniform lane, to every lane (float overload; the double one is above)
 __device__ __forceinline__ float readlane_d(float v, int lane)
 {
