@@ -108,6 +108,74 @@ bool is_dir(const string &p)
 
 }  // namespace
 
+// ---- symbolic ODEs and Jacobian (verbose >= 2 / >= 3) ------------------------
+// printOdes (Simulation.cuh:83-147): one line per non-fed species; a fed
+// species prints nothing.  Terms follow the net-stoichiometry entries
+// (reaction ascending), each "+c*k[r]" (or "-c*k[r]") followed by one "*x[s]"
+// per unit of reactant order.
+void print_odes(int S, const vector<int32_t> &lptr, const vector<int32_t> &lsp, const vector<int32_t> &lexp,
+                const vector<int32_t> &hptr, const vector<int32_t> &hrx, const vector<int32_t> &hcoef,
+                const vector<double> &feed)
+{
+    for (int g = 0; g < S; g++) {
+        if (feed[g] != 0) continue;
+        string s;
+        for (int e = hptr[g]; e < hptr[g + 1]; e++) {
+            const int r = hrx[e], c = hcoef[e];
+            string term = (c > 0 ? "+" : "") + std::to_string(c) + "*k[" + std::to_string(r) + "]";
+            for (int j = lptr[r]; j < lptr[r + 1]; j++)
+                for (int l = 0; l < lexp[j]; l++) term += "*x[" + std::to_string(lsp[j]) + "]";
+            s += term;
+        }
+        if (s.empty())
+            printf("dx[%d] = 0\n", g);
+        else
+            printf("dx[%d] = %s\n", g, s.c_str());
+    }
+}
+
+// printJac / printJac_ (Simulation.cuh:149-228): every entry (i, g), row
+// major.  A column g of a fed species is 0.  For a reactant g of order 1 the
+// term is "+c*k[r]" times the other reactants; of order e > 1 it is
+// "(c*e)*k[r]" (no explicit "+"), x[g] repeated e-1 times, then the other
+// reactants.  Other reactants appear once each, whatever their order.
+void print_jac(int S, const vector<int32_t> &lptr, const vector<int32_t> &lsp, const vector<int32_t> &lexp,
+               const vector<int32_t> &hptr, const vector<int32_t> &hrx, const vector<int32_t> &hcoef,
+               const vector<double> &feed)
+{
+    for (int i = 0; i < S; i++) {
+        for (int g = 0; g < S; g++) {
+            if (feed[g] != 0) {
+                printf("dx[%d][%d] = 0\n", i, g);
+                continue;
+            }
+            string s;
+            for (int e = hptr[i]; e < hptr[i + 1]; e++) {
+                const int r = hrx[e], c = hcoef[e];
+                for (int j = lptr[r]; j < lptr[r + 1]; j++) {
+                    if (lsp[j] != g) continue;
+                    string term;
+                    if (lexp[j] == 1) {
+                        term = (c > 0 ? "+" : "") + std::to_string(c) + "*k[" + std::to_string(r) + "]";
+                    } else if (lexp[j] > 1) {
+                        term = std::to_string(c * lexp[j]) + "*k[" + std::to_string(r) + "]";
+                        for (int l = 0; l < lexp[j] - 1; l++) term += "*x[" + std::to_string(g) + "]";
+                    } else {
+                        continue;
+                    }
+                    for (int jj = lptr[r]; jj < lptr[r + 1]; jj++)
+                        if (lsp[jj] != g) term += "*x[" + std::to_string(lsp[jj]) + "]";
+                    s += term;
+                }
+            }
+            if (s.empty())
+                printf("dx[%d][%d] = 0\n", i, g);
+            else
+                printf("dx[%d][%d] = %s\n", i, g, s.c_str());
+        }
+    }
+}
+
 int main(int argc, char *argv[])
 {
     const auto t_start = std::chrono::steady_clock::now();
@@ -273,6 +341,21 @@ int main(int argc, char *argv[])
         hcoef(std::max(nh, 1));
     lassie_csr_from_dense(S, R, Ld.data(), Rd.data(), nullptr, nullptr, lptr.data(), lsp.data(), lexp.data(),
                           hptr.data(), hrx.data(), hcoef.data());
+
+    if (verbose > 1) {
+        std::cout << "\n***************************************************************";
+        std::cout << "\nODEs:\n";
+        print_odes(S, lptr, lsp, lexp, hptr, hrx, hcoef, mfeed);
+        std::cout << "***************************************************************\n";
+        std::cout << "\n";
+    }
+    if (verbose > 2) {
+        std::cout << "\n***************************************************************";
+        std::cout << "\nJacobian:\n";
+        print_jac(S, lptr, lsp, lexp, hptr, hrx, hcoef, mfeed);
+        std::cout << "***************************************************************\n";
+        std::cout << "\n";
+    }
 
     // SpeciesSampling (Simulation.cuh:451-465)
     {

@@ -115,3 +115,35 @@ def test_stochastic_conversion_matches_oracle(tmp_path, gpu_lib):
     m_conc = synth.model_from_dense(m.L, m.Rm, k, (m.x0 * vol * avo) / conv)
     o_out, _, o_rc, _, _ = O.solve(m_conc, t)
     assert (tmp_path / "out" / "output" / "Solution").read_text() == _fmt(t, o_out * (vol * avo))
+
+
+def test_verbose_prints_odes_and_jacobian(tmp_path):
+    """-v3 prints the symbolic ODEs and Jacobian before any GPU work
+    (Simulation.cuh:433-449).  Model: 2A -> B (k0), A + B -> C (k1), 0 -> A (k2);
+    expected text written out by hand from printOdes / printJac_ (a species of
+    order 2 contributes "(c*2)*k[r]*x[g]" with no explicit "+")."""
+    m = cases.from_reactions(3, [({0: 2}, {1: 1}), ({0: 1, 1: 1}, {2: 1}), ({}, {0: 1})], [1.0, 2.0, 3.0],
+                             [1.0, 0.5, 0.0])
+    d = model_folder(tmp_path, m)
+    rc, out = run(str(d), str(tmp_path / "out"), "-v3")
+    odes = ["dx[0] = -2*k[0]*x[0]*x[0]-1*k[1]*x[0]*x[1]+1*k[2]",
+            "dx[1] = +1*k[0]*x[0]*x[0]-1*k[1]*x[0]*x[1]",
+            "dx[2] = +1*k[1]*x[0]*x[1]"]
+    jac = ["dx[0][0] = -4*k[0]*x[0]-1*k[1]*x[1]", "dx[0][1] = -1*k[1]*x[0]", "dx[0][2] = 0",
+           "dx[1][0] = 2*k[0]*x[0]-1*k[1]*x[1]", "dx[1][1] = -1*k[1]*x[0]", "dx[1][2] = 0",
+           "dx[2][0] = +1*k[1]*x[1]", "dx[2][1] = +1*k[1]*x[0]", "dx[2][2] = 0"]
+    lines = out.splitlines()
+    i = lines.index("ODEs:")
+    assert lines[i + 1:i + 4] == odes
+    j = lines.index("Jacobian:")
+    assert lines[j + 1:j + 10] == jac
+
+
+def test_fed_species_print_no_ode_and_zero_jacobian_columns(tmp_path):
+    m = cases.with_feed()
+    d = model_folder(tmp_path, m)
+    rc, out = run(str(d), str(tmp_path / "out"), "-v3")
+    lines = out.splitlines()
+    i = lines.index("ODEs:")
+    assert not any(l.startswith("dx[1] =") for l in lines[i + 1:i + 4])  # species 1 is fed
+    assert all(f"dx[{r}][1] = 0" in lines for r in range(3))
