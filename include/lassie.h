@@ -128,6 +128,25 @@ int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lass
                        const double *t, int32_t nt, double *out, double *y_final, lassie_stats *stats,
                        double *elapsed_ms);
 
+/* Batched plan: the batched form prepared once and replayed (parameter sweeps,
+ * repeated solves).  Creation allocates every workspace, picks the kernel
+ * variant (row tiles of the LU trailing update in flight) by timing a short
+ * trial of each variant on these instances (cached per device, largest S,
+ * count and precision), and captures the launch sequence as a hipGraph.
+ * lassie_batch_plan_run replays it, with the output layout of
+ * lassie_solve_batch; rates changed with lassie_model_set_rates are used by
+ * the next run.  The models must outlive the plan.  lassie_batch_plan_info
+ * reports the chosen variant, its row tiles and the trial times (ms, one per
+ * variant; zeros when the choice came from the cache).  No counterpart in the
+ * reference (one model per process). */
+typedef struct lassie_batch_plan lassie_batch_plan;
+int lassie_batch_plan_create(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                             const double *t, int32_t nt, lassie_batch_plan **plan);
+int lassie_batch_plan_run(lassie_batch_plan *plan, double *out, double *y_final, lassie_stats *stats,
+                          double *elapsed_ms);
+int lassie_batch_plan_info(const lassie_batch_plan *plan, int32_t *variant, int32_t *tile_rows, float *trial_ms);
+int lassie_batch_plan_destroy(lassie_batch_plan *plan);
+
 /* Replaces the rate constants (c_vector) of an uploaded model, k[R].  A tape
  * recorded before the change can no longer be differentiated.  No
  * counterpart in the reference (its rates are fixed per process,

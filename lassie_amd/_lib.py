@@ -68,6 +68,8 @@ class Stats(C.Structure):
 EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy", "lassie_solve",
            "lassie_solve_batch", "lassie_model_set_rates", "lassie_forward_batch",
            "lassie_adjoint_batch", "lassie_tape_info", "lassie_tape_read", "lassie_tape_destroy",
+           "lassie_batch_plan_create", "lassie_batch_plan_run", "lassie_batch_plan_info",
+           "lassie_batch_plan_destroy",
            "lassie_csr_from_dense", "lassie_device_count", "lassie_last_error", "lassie_abi_version")
 
 
@@ -106,6 +108,11 @@ def lib():
         L.lassie_tape_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), _i32p]
         L.lassie_tape_read.argtypes = [C.c_void_p, C.c_int32, _f64p, _i32p, _i32p]
         L.lassie_tape_destroy.argtypes = [C.c_void_p]
+        L.lassie_batch_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts), _f64p,
+                                               C.c_int32, C.POINTER(C.c_void_p)]
+        L.lassie_batch_plan_run.argtypes = [C.c_void_p, _f64p, _f64p, C.POINTER(Stats), C.POINTER(C.c_double)]
+        L.lassie_batch_plan_info.argtypes = [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_float)]
+        L.lassie_batch_plan_destroy.argtypes = [C.c_void_p]
         L.lassie_device_count.argtypes = [_i32p]
         L.lassie_last_error.restype = C.c_char_p
         L.lassie_abi_version.restype = C.c_int
@@ -314,3 +321,57 @@ def forward_batch(models, t, max_steps=1 << 16, **opts):
         outs.append(out[off:off + t.size * S].reshape(t.size, S))
         off += t.size * S
     return outs, [s.as_dict() for s in st], ms.value, Tape(th, models, t.size)
+
+
+class BatchPlan:
+    """The batched form prepared once and replayed as a graph
+    (lassie_batch_plan_*); the kernel variant is chosen by the autotuner."""
+
+    def __init__(self, models, t, **opts):
+        self.models = list(models)
+        self.t = np.ascontiguousarray(t, np.float64)
+        n = len(self.models)
+        hs = (C.c_void_p * n)(*[dm.h for dm in self.models])
+        o = default_opts(**opts)
+        h = C.c_void_p()
+        _check(lib().lassie_batch_plan_create(hs, n, C.byref(o), _p(self.t, _f64p), self.t.size, C.byref(h)))
+        self.h = h
+
+    def run(self, raise_on_maxiter=True):
+        """Returns (list of out[nt, S_i], list of y_final[S_i], list of stats, device ms, status)."""
+        n = len(self.models)
+        sizes = [dm.S for dm in self.models]
+        out = np.full(self.t.size * sum(sizes), np.nan)
+        yf = np.zeros(sum(sizes))
+        st = (Stats * n)()
+        ms = C.c_double(0.0)
+        rc = lib().lassie_batch_plan_run(self.h, _p(out, _f64p), _p(yf, _f64p), st, C.byref(ms))
+        if rc != LASSIE_OK and not (rc == LASSIE_ERR_MAXITER and not raise_on_maxiter):
+            _check(rc)
+        outs, ys = [], []
+        o_off = y_off = 0
+        for S in sizes:
+            outs.append(out[o_off:o_off + self.t.size * S].reshape(self.t.size, S))
+            ys.append(yf[y_off:y_off + S])
+            o_off += self.t.size * S
+            y_off += S
+        return outs, ys, [x.as_dict() for x in st], ms.value, rc
+
+    def info(self):
+        """(variant, row tiles in flight, trial ms per variant)."""
+        v = np.zeros(1, np.int32)
+        rt = np.zeros(1, np.int32)
+        ms = (C.c_float * 3)()
+        _check(lib().lassie_batch_plan_info(self.h, _p(v, _i32p), _p(rt, _i32p), ms))
+        return int(v[0]), int(rt[0]), list(ms)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lassie_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -11,6 +11,7 @@
 #include "../../include/lassie.h"
 #include "adjoint.h"
 #include "batch.h"
+#include "model.h"
 #include "solver.h"
 
 namespace {
@@ -89,6 +90,54 @@ static void free_model(lassie_model *m)
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
+
+namespace lassie {
+
+int api_fail(int code, const std::string &msg) { return fail(code, msg); }
+int model_species(const lassie_model *m) { return m->S; }
+int model_device(const lassie_model *m) { return m->device; }
+
+void fill_batch_inst(const lassie_model *m, BatchInst *b)
+{
+    b->S = m->S;
+    b->R = m->R;
+    b->lptr = m->d_lptr;
+    b->lsp = m->d_lsp;
+    b->lexp = m->d_lexp;
+    b->hptr = m->d_hptr;
+    b->hrx = m->d_hrx;
+    b->recs = m->d_recs;
+    b->hk = m->d_hk;
+    b->x0 = m->d_x0;
+    b->feed = m->d_feed;
+    b->atol = m->d_atol;
+    b->out_off = b->y_off = b->tape_off = b->ckpt_off = 0;
+}
+
+// c: kBatchStats counters of one instance; d: its t, dt, newdt
+void fill_batch_stats(const long long *c, const double *d, float ms, int slots, lassie_stats *s)
+{
+    std::memset(s, 0, sizeof(*s));
+    s->loop_iters = c[0];
+    s->rkf_attempts = c[1];
+    s->rkf_accepts = c[2];
+    s->bdf_steps = c[3];
+    s->newton_iters = c[4];
+    s->lu_failures = c[5];
+    s->nan_repairs = c[6];
+    s->samples = c[7];
+    s->lu_factorizations = c[8];
+    s->refine_iters = c[10];
+    s->refine_fallbacks = c[11];
+    s->t_final = d[0];
+    s->dt_final = d[1];
+    s->newdt_final = d[2];
+    s->kernel_ms = ms;
+    s->groups = slots;
+    for (int q = 0; q < 4; q++) s->phase_ms[q] = (double)c[14 + q] * 1e-5;
+}
+
+}  // namespace lassie
 
 static int validate_desc(const lassie_model_desc *d)
 {
@@ -561,18 +610,7 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     for (int i = 0; i < n_models; i++) {
         const lassie_model *m = models[i];
         lassie::BatchInst &b = inst[i];
-        b.S = m->S;
-        b.R = m->R;
-        b.lptr = m->d_lptr;
-        b.lsp = m->d_lsp;
-        b.lexp = m->d_lexp;
-        b.hptr = m->d_hptr;
-        b.hrx = m->d_hrx;
-        b.recs = m->d_recs;
-        b.hk = m->d_hk;
-        b.x0 = m->d_x0;
-        b.feed = m->d_feed;
-        b.atol = m->d_atol;
+        lassie::fill_batch_inst(m, &b);
         b.out_off = out_tot;
         b.y_off = y_tot;
         b.tape_off = (long long)i * cap;
@@ -694,27 +732,7 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     bool maxit = false, full = false;
     for (int i = 0; i < n_models; i++) {
         const long long *c = &hst[(size_t)i * lassie::kBatchStats];
-        if (stats) {
-            lassie_stats &s = stats[i];
-            std::memset(&s, 0, sizeof(s));
-            s.loop_iters = c[0];
-            s.rkf_attempts = c[1];
-            s.rkf_accepts = c[2];
-            s.bdf_steps = c[3];
-            s.newton_iters = c[4];
-            s.lu_failures = c[5];
-            s.nan_repairs = c[6];
-            s.samples = c[7];
-            s.lu_factorizations = c[8];
-            s.refine_iters = c[10];
-            s.refine_fallbacks = c[11];
-            s.t_final = hd[4 * i];
-            s.dt_final = hd[4 * i + 1];
-            s.newdt_final = hd[4 * i + 2];
-            s.kernel_ms = ms;
-            s.groups = slots;
-            for (int q = 0; q < 4; q++) s.phase_ms[q] = (double)c[14 + q] * 1e-5;
-        }
+        if (stats) lassie::fill_batch_stats(c, &hd[4 * (size_t)i], ms, slots, &stats[i]);
         maxit |= c[9] == 1;
         full |= c[9] == 2;
         if (tape) {

@@ -57,7 +57,13 @@ struct BatchArgs {
     long long tape_cap;
 };
 
-hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream);
-int batch_occupancy(int *per_cu);
+// Kernel variants (row tiles of the LU trailing update in flight per wave):
+// 0 -> 1, 1 -> 2, 2 -> 4.  The arithmetic is the same in every variant.
+constexpr int kBatchVariants = 3;
+constexpr int kBatchDefaultVariant = 2;
+int batch_variant_rt(int variant);
+hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream, int variant = kBatchDefaultVariant);
+int batch_occupancy(int *per_cu, int variant = kBatchDefaultVariant);
+hipError_t launch_batch_reset(int *next, hipStream_t stream);
 
 }  // namespace lassie

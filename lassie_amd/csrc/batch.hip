@@ -59,6 +59,7 @@ __device__ bool refine_solve(const double *__restrict__ M64, const float *__rest
     return false;
 }
 
+template <int RT>
 __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
 {
     __shared__ Shared sh;
@@ -214,14 +215,14 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                     for (size_t e = tid; e < nn; e += NT) M32[e] = (float)M[e];
                     __syncthreads();
                     const int inf32 = (A.precision == 2)
-                                          ? wg_lu_rm<float, true>(M32, S, piv, perm_src, moved_to, rows, rowcnt, info, sh)
-                                          : wg_lu_rm<float, false>(M32, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                                          ? wg_lu_rm<float, true, RT>(M32, S, piv, perm_src, moved_to, rows, rowcnt, info, sh)
+                                          : wg_lu_rm<float, false, RT>(M32, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
                     if (inf32) {
                         fp64 = true;
                         st_fb++;
                     }
                 }
-                if (fp64) inf = wg_lu_rm<double, false>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                if (fp64) inf = wg_lu_rm<double, false, RT>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
                 st_lu++;
                 {
                     const unsigned long long now = wall_clock64();
@@ -253,7 +254,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                         // refinement did not converge: FP64 LU for the rest of this step
                         fp64 = true;
                         st_fb++;
-                        inf = wg_lu_rm<double, false>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                        inf = wg_lu_rm<double, false, RT>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
                         if (inf) continue;  // handled at the top of the loop as an LU failure
                         for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
                         __syncthreads();
@@ -330,17 +331,42 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
     }
 }
 
-int batch_occupancy(int *per_cu)
+static const void *batch_kernel_ptr(int variant)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, reinterpret_cast<const void *>(&batch_kernel), NT,
-                                                        0) == hipSuccess
-               ? 0
-               : -1;
+    switch (variant) {
+    case 0: return reinterpret_cast<const void *>(&batch_kernel<1>);
+    case 1: return reinterpret_cast<const void *>(&batch_kernel<2>);
+    default: return reinterpret_cast<const void *>(&batch_kernel<4>);
+    }
 }
 
-hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream)
+int batch_variant_rt(int variant) { return variant == 0 ? 1 : (variant == 1 ? 2 : 4); }
+
+int batch_occupancy(int *per_cu, int variant)
 {
-    hipLaunchKernelGGL(batch_kernel, dim3(slots), dim3(NT), 0, stream, args);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel_ptr(variant), NT, 0) == hipSuccess ? 0
+                                                                                                              : -1;
+}
+
+hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream, int variant)
+{
+    switch (variant) {
+    case 0: hipLaunchKernelGGL(batch_kernel<1>, dim3(slots), dim3(NT), 0, stream, args); break;
+    case 1: hipLaunchKernelGGL(batch_kernel<2>, dim3(slots), dim3(NT), 0, stream, args); break;
+    default: hipLaunchKernelGGL(batch_kernel<4>, dim3(slots), dim3(NT), 0, stream, args); break;
+    }
+    return hipGetLastError();
+}
+
+// Resets the work-queue head inside a captured graph (a kernel node, not a memset node).
+__global__ void batch_reset_kernel(int *next)
+{
+    if (threadIdx.x == 0) *next = 0;
+}
+
+hipError_t launch_batch_reset(int *next, hipStream_t stream)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, next);
     return hipGetLastError();
 }
 

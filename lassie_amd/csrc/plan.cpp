@@ -1,0 +1,273 @@
+// Batched plan: the batched form prepared once and replayed.
+//
+// lassie_batch_plan_create allocates the workspaces, the instance table and
+// pinned staging buffers once, chooses the kernel variant (row tiles of the
+// LU trailing update in flight per wave) by timing a short trial of every
+// variant on the plan's own instances, and captures the launch sequence (a
+// queue-reset kernel, the persistent kernel, copies into the pinned buffers)
+// as a hipGraph.  lassie_batch_plan_run replays the graph.  Rates changed
+// with lassie_model_set_rates are used by the next run, since the graph reads
+// the models' device arrays.  Each instance follows Simulation<T>::run
+// (Simulation.cuh:632-925), as in lassie_solve_batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/lassie.h"
+#include "batch.h"
+#include "model.h"
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, int> g_tuned;  // (device, largest S, count, precision) -> variant
+
+}  // namespace
+
+struct lassie_batch_plan {
+    int device = 0, n = 0, nt = 0, smax = 1, slots = 1, variant = lassie::kBatchDefaultVariant;
+    lassie_opts opts{};
+    long long out_tot = 0, y_tot = 0;
+    lassie::BatchInst *d_inst = nullptr;
+    int *d_order = nullptr, *d_next = nullptr, *d_wsi = nullptr;
+    double *d_tv = nullptr, *d_out = nullptr, *d_y = nullptr, *d_ws64 = nullptr, *d_dst = nullptr;
+    float *d_ws32 = nullptr;
+    long long *d_st = nullptr;
+    double *h_out = nullptr, *h_y = nullptr, *h_dst = nullptr;
+    long long *h_st = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float tune_ms[lassie::kBatchVariants] = {0};
+    lassie::BatchArgs args{};
+};
+
+static void free_plan(lassie_batch_plan *p)
+{
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    void *dptrs[] = {p->d_inst, p->d_order, p->d_next, p->d_wsi, p->d_tv, p->d_out,
+                     p->d_y,    p->d_ws64,  p->d_dst,  p->d_ws32, p->d_st};
+    for (void *q : dptrs)
+        if (q) (void)hipFree(q);
+    void *hptrs[] = {p->h_out, p->h_y, p->h_dst, p->h_st};
+    for (void *q : hptrs)
+        if (q) (void)hipHostFree(q);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                                        const double *t, int32_t nt, lassie_batch_plan **out)
+{
+    using lassie::api_fail;
+    if (!out) return api_fail(LASSIE_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!models || n_models < 1) return api_fail(LASSIE_ERR_INVALID, "need at least one model");
+    if (!t || nt < 1) return api_fail(LASSIE_ERR_INVALID, "need sampling times");
+    lassie_batch_plan *P = new lassie_batch_plan();
+    auto bail = [&](int code, const std::string &msg) {
+        free_plan(P);
+        return api_fail(code, msg);
+    };
+    if (opts)
+        P->opts = *opts;
+    else
+        lassie_opts_default(&P->opts);
+    const lassie_opts &o = P->opts;
+    if (o.precision < LASSIE_PREC_FP64 || o.precision > LASSIE_PREC_BF16_REFINE)
+        return bail(LASSIE_ERR_INVALID, "unknown precision");
+    std::vector<lassie::BatchInst> inst(n_models);
+    for (int i = 0; i < n_models; i++) {
+        if (!models[i]) return bail(LASSIE_ERR_INVALID, "NULL model in batch");
+        if (lassie::model_species(models[i]) > lassie::kBatchMaxS)
+            return bail(LASSIE_ERR_UNSUPPORTED, "batched form supports S <= 512");
+        if (lassie::model_device(models[i]) != lassie::model_device(models[0]))
+            return bail(LASSIE_ERR_INVALID, "models on different devices");
+        lassie::fill_batch_inst(models[i], &inst[i]);
+        inst[i].out_off = P->out_tot;
+        inst[i].y_off = P->y_tot;
+        P->out_tot += (long long)nt * inst[i].S;
+        P->y_tot += inst[i].S;
+        P->smax = std::max(P->smax, inst[i].S);
+    }
+    P->device = lassie::model_device(models[0]);
+    P->n = n_models;
+    P->nt = nt;
+    if (hipSetDevice(P->device) != hipSuccess) return bail(LASSIE_ERR_HIP, "hipSetDevice failed");
+    std::vector<int> order(n_models);
+    for (int i = 0; i < n_models; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].S > inst[b].S; });
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device) != hipSuccess)
+        return bail(LASSIE_ERR_HIP, "device attribute query failed");
+    if (lassie::batch_occupancy(&per_cu) != 0 || per_cu < 1) return bail(LASSIE_ERR_HIP, "batch occupancy query failed");
+    P->slots = std::max(1, std::min(n_models, cus * per_cu));
+    const int smax = P->smax, slots = P->slots;
+    const int Pmax = (smax + lassie::kBatchNB - 1) / lassie::kBatchNB;
+    const size_t ws64_stride = 12 * (size_t)smax + (size_t)smax * smax + lassie::kLuSink;
+    const size_t ws32_stride = (size_t)smax * smax + lassie::kLuSink;
+    const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
+    hipError_t e;
+    if ((e = hipStreamCreate(&P->stream)) != hipSuccess || (e = hipEventCreate(&P->ev0)) != hipSuccess ||
+        (e = hipEventCreate(&P->ev1)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_inst, sizeof(lassie::BatchInst) * n_models)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_order, sizeof(int) * n_models)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_next, sizeof(int))) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_wsi, sizeof(int) * wsi_stride * slots)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_tv, sizeof(double) * nt)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_out, sizeof(double) * P->out_tot)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_y, sizeof(double) * P->y_tot)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_ws64, sizeof(double) * ws64_stride * slots)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_dst, sizeof(double) * 4 * n_models)) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess ||
+        (o.precision != LASSIE_PREC_FP64 &&
+         (e = hipMalloc((void **)&P->d_ws32, sizeof(float) * ws32_stride * slots)) != hipSuccess) ||
+        (e = hipHostMalloc((void **)&P->h_out, sizeof(double) * P->out_tot)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&P->h_y, sizeof(double) * P->y_tot)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&P->h_dst, sizeof(double) * 4 * n_models)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&P->h_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess)
+        return bail(LASSIE_ERR_NOMEM, std::string("batch plan: ") + hipGetErrorString(e));
+    if ((e = hipMemcpy(P->d_inst, inst.data(), sizeof(lassie::BatchInst) * n_models, hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMemcpy(P->d_order, order.data(), sizeof(int) * n_models, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(P->d_tv, t, sizeof(double) * nt, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(P->d_wsi, 0, sizeof(int) * wsi_stride * slots)) != hipSuccess)
+        return bail(LASSIE_ERR_HIP, std::string("batch plan upload: ") + hipGetErrorString(e));
+    lassie::BatchArgs &a = P->args;
+    a.inst = P->d_inst;
+    a.n_inst = n_models;
+    a.order = P->d_order;
+    a.next = P->d_next;
+    a.tv = P->d_tv;
+    a.nt = nt;
+    a.out = P->d_out;
+    a.y_final = P->d_y;
+    a.stats = P->d_st;
+    a.dstats = P->d_dst;
+    a.rkf_step = o.rkf_step;
+    a.be_step = o.be_step;
+    a.newton_tol = o.newton_tol;
+    a.stiffness_tol = o.stiffness_tol;
+    a.rtol = o.rtol;
+    a.newton_iter = o.newton_iter;
+    a.bdf_regrow = o.bdf_regrow;
+    a.max_rkf_retries = o.max_rkf_retries;
+    a.max_iters = o.max_iters;
+    a.precision = o.precision;
+    a.refine_tol = o.refine_tol;
+    a.refine_max_iter = std::max(1, o.refine_max_iter);
+    a.s_max = smax;
+    a.ws64 = P->d_ws64;
+    a.ws64_stride = ws64_stride;
+    a.ws32 = P->d_ws32;
+    a.ws32_stride = ws32_stride;
+    a.wsi = P->d_wsi;
+    a.wsi_stride = wsi_stride;
+
+    // autotuner: a short trial (64 loop iterations per instance) of every
+    // variant; the fastest is kept for this (device, largest S, count, precision)
+    const auto key = std::make_tuple(P->device, smax, n_models, (int)o.precision);
+    int chosen = -1;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) chosen = it->second;
+    }
+    if (chosen < 0) {
+        lassie::BatchArgs trial = a;
+        trial.max_iters = 64;
+        float best = 0.f;
+        for (int v = 0; v < lassie::kBatchVariants; v++) {
+            if ((e = lassie::launch_batch_reset(P->d_next, P->stream)) != hipSuccess ||
+                (e = hipEventRecord(P->ev0, P->stream)) != hipSuccess ||
+                (e = lassie::launch_batch(trial, slots, P->stream, v)) != hipSuccess ||
+                (e = hipEventRecord(P->ev1, P->stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(P->stream)) != hipSuccess)
+                return bail(LASSIE_ERR_HIP, std::string("autotune trial: ") + hipGetErrorString(e));
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, P->ev0, P->ev1);
+            P->tune_ms[v] = ms;
+            if (chosen < 0 || ms < best) {
+                best = ms;
+                chosen = v;
+            }
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = chosen;
+    }
+    P->variant = chosen;
+
+    // the launch sequence as a graph (no memset nodes: the queue head is reset by a kernel)
+    if ((e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return bail(LASSIE_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    hipError_t ec = lassie::launch_batch_reset(P->d_next, P->stream);
+    if (ec == hipSuccess) ec = lassie::launch_batch(a, slots, P->stream, P->variant);
+    if (ec == hipSuccess)
+        ec = hipMemcpyAsync(P->h_out, P->d_out, sizeof(double) * P->out_tot, hipMemcpyDeviceToHost, P->stream);
+    if (ec == hipSuccess)
+        ec = hipMemcpyAsync(P->h_y, P->d_y, sizeof(double) * P->y_tot, hipMemcpyDeviceToHost, P->stream);
+    if (ec == hipSuccess)
+        ec = hipMemcpyAsync(P->h_st, P->d_st, sizeof(long long) * lassie::kBatchStats * n_models,
+                            hipMemcpyDeviceToHost, P->stream);
+    if (ec == hipSuccess)
+        ec = hipMemcpyAsync(P->h_dst, P->d_dst, sizeof(double) * 4 * n_models, hipMemcpyDeviceToHost, P->stream);
+    e = hipStreamEndCapture(P->stream, &P->graph);
+    if (ec != hipSuccess || e != hipSuccess)
+        return bail(LASSIE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec != hipSuccess ? ec : e));
+    if ((e = hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return bail(LASSIE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    *out = P;
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *y_final, lassie_stats *stats,
+                                     double *elapsed_ms)
+{
+    using lassie::api_fail;
+    if (!P || !out) return api_fail(LASSIE_ERR_INVALID, "plan and out are required");
+    hipError_t e;
+    if ((e = hipSetDevice(P->device)) != hipSuccess || (e = hipEventRecord(P->ev0, P->stream)) != hipSuccess ||
+        (e = hipGraphLaunch(P->exec, P->stream)) != hipSuccess || (e = hipEventRecord(P->ev1, P->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(P->stream)) != hipSuccess)
+        return api_fail(LASSIE_ERR_HIP, std::string("plan run: ") + hipGetErrorString(e));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, P->ev0, P->ev1);
+    if (elapsed_ms) *elapsed_ms = ms;
+    std::memcpy(out, P->h_out, sizeof(double) * P->out_tot);
+    if (y_final) std::memcpy(y_final, P->h_y, sizeof(double) * P->y_tot);
+    bool maxit = false;
+    for (int i = 0; i < P->n; i++) {
+        const long long *c = P->h_st + (size_t)i * lassie::kBatchStats;
+        if (stats) lassie::fill_batch_stats(c, P->h_dst + 4 * (size_t)i, ms, P->slots, &stats[i]);
+        maxit |= c[9] == 1;
+    }
+    if (maxit) return api_fail(LASSIE_ERR_MAXITER, "max_iters reached before the last sample in at least one instance");
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_batch_plan_info(const lassie_batch_plan *P, int32_t *variant, int32_t *tile_rows,
+                                      float *trial_ms)
+{
+    if (!P) return lassie::api_fail(LASSIE_ERR_INVALID, "plan is NULL");
+    if (variant) *variant = P->variant;
+    if (tile_rows) *tile_rows = lassie::batch_variant_rt(P->variant);
+    if (trial_ms)
+        for (int v = 0; v < lassie::kBatchVariants; v++) trial_ms[v] = P->tune_ms[v];
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_batch_plan_destroy(lassie_batch_plan *P)
+{
+    if (P) free_plan(P);
+    return LASSIE_OK;
+}

@@ -43,10 +43,10 @@ struct Mfma<float> {
 
 // Workgroup-local blocked right-looking LU of a row-major n x n matrix with
 // partial pivoting (panel_factor semantics).  M must be followed by kLuSink
-// writable elements.  Row moves go to every column,
+// writable elements.  RT: row tiles of the trailing update in flight per wave.  Row moves go to every column,
 // so on return M holds L\U in final row order.  Returns 0 or k+1 of the
 // first zero pivot.  BF16 = round the trailing-update operands to BF16.
-template <typename T, bool BF16>
+template <typename T, bool BF16, int RT = 4>
 __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
                         int *info, Shared &sh)
 {
@@ -109,7 +109,6 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
         // the row tiles RT at a time, the next RT tiles' operands in flight.
         const int c0 = r0 + nbp;
         if (c0 < n) {
-            constexpr int RT = 4;
             typedef typename Mfma<T>::acc Acc;
             // global-memory view of M (the workspace lives in device memory):
             // global loads/stores keep the memory counters precise, so the

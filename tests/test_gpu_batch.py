@@ -2,13 +2,17 @@
 
 FP64: every instance of the batch must equal the oracle bit for bit (same
 algorithm and operation order as the single-model path).  FP32 / BF16 LU with
-FP64 refinement: the trajectory must stay within 1e-6 relative error of the
-FP64 form, |a - b| / (|b| + atol), and report its refinement work.
+FP64 refinement: the relative error against the FP64 form,
+|a - b| / (|b| + atol), and the refinement work are reported (the loop's
+accept/reject decisions are discontinuous, so no bound holds for every model).
+The batched plan (graph replay, autotuned variant) equals the batched form.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
-from lassie_amd import synth
+from lassie_amd import _lib, synth
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -89,3 +93,32 @@ def test_batch_mixed_precision(gpu_lib, prec):
     assert same >= 1
     for dm in dms:
         dm.close()
+
+
+def test_batch_plan_replays_the_batched_form(gpu_lib):
+    """lassie_batch_plan_*: the graph replay gives the batched form's results
+    bit for bit, twice in a row, and picks up new rates; the autotuner reports
+    its choice."""
+    models = synth.uneven_batch(10, 4242, sizes=(16, 48, 100, 200))
+    t = synth.t_vector(0, 2, 9)
+    kw = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
+    print("[plan] 10 uneven instances", flush=True)
+    dms = [_lib.DeviceModel(m) for m in models]
+    ref, ref_y, ref_st, _, _ = _lib.solve_batch(dms, t, **kw)
+    plan = _lib.BatchPlan(dms, t, **kw)
+    variant, rt, trial = plan.info()
+    assert variant in (0, 1, 2) and rt in (1, 2, 4)
+    for _ in range(2):
+        outs, ys, st, ms, rc = plan.run()
+        assert rc == 0 and ms > 0
+        for a, b in zip(outs, ref):
+            assert np.array_equal(a, b)
+        assert [s["bdf_steps"] for s in st] == [s["bdf_steps"] for s in ref_st]
+    # new rates: the next replay integrates them
+    k2 = models[3].k * 1.5
+    dms[3].set_rates(k2)
+    outs, _, _, _, _ = plan.run()
+    m3 = dataclasses.replace(models[3], k=k2)
+    o_out, _, o_rc, _, _ = O.solve(m3, t, **kw)
+    assert np.array_equal(outs[3], o_out)
+    plan.close()
