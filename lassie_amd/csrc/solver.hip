@@ -145,6 +145,24 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
     __syncthreads();
 }
 
+// panel_factor with the fewest rows per thread that cover the panel's rows
+// [p*NBC, n): later panels have fewer live rows, and the register panel is
+// processed without per-row control flow, so a smaller RPT is less work.  The
+// rows each thread holds, and so every result, are the same for any RPT that
+// covers them.
+template <int RPT, int NBC>
+__device__ void panel_fit(double *M, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
+                          int *rowcnt, int *info, Shared &sh)
+{
+    if constexpr (RPT > 1) {
+        if (n - p * NBC <= NT * (RPT / 2)) {
+            panel_fit<RPT / 2, NBC>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+            return;
+        }
+    }
+    panel_factor<RPT, NBC, double, false>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+}
+
 // Blocked right-looking LU over all groups; strips are owned round-robin.
 // The owner of strip p+1 updates it first and factors panel p+1 at once
 // (look-ahead), so each panel costs one grid barrier; left strips receive the
@@ -180,8 +198,8 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
             strip_update<NBC>(A.M, n, p, q, A.perm_src, A.moved_to, rows, cnt, sh);
             if (q == p + 1) {
                 __syncthreads();
-                panel_factor<RPT, NBC, double, false>(A.M, n, p + 1, A.piv, A.perm_src, A.moved_to,
-                                       A.rows + (size_t)(p + 1) * n, A.rowcnt + p + 1, &A.ctl->info, sh);
+                panel_fit<RPT, NBC>(A.M, n, p + 1, A.piv, A.perm_src, A.moved_to, A.rows + (size_t)(p + 1) * n,
+                                    A.rowcnt + p + 1, &A.ctl->info, sh);
                 __syncthreads();
             }
         }
