@@ -381,7 +381,7 @@ int lassie_model_create(const lassie_model_desc *d, lassie_model **out)
             fail(LASSIE_ERR_NOMEM, std::string("triangular-solve buffers: ") + hipGetErrorString(e));
             return bail(LASSIE_ERR_NOMEM);
         }
-        if ((e = hipMalloc((void **)&m->d_M, sizeof(double) * (size_t)S * S)) != hipSuccess) {
+        if ((e = hipMalloc((void **)&m->d_M, sizeof(double) * ((size_t)S * S + lassie::kLuSink))) != hipSuccess) {
             fail(LASSIE_ERR_NOMEM, std::string("dense Newton matrix: ") + hipGetErrorString(e));
             return bail(LASSIE_ERR_NOMEM);
         }

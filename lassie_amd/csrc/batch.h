@@ -8,7 +8,6 @@ namespace lassie {
 
 constexpr int kBatchMaxS = 512;  // largest instance of the batched form
 constexpr int kBatchNB = 16;     // panel width of the workgroup-local LU
-constexpr int kLuSink = 64;      // elements after each workgroup matrix that take discarded stores
 
 // One instance: device pointers of an uploaded model and where its outputs go.
 struct BatchInst {

@@ -8,6 +8,7 @@ namespace lassie {
 
 constexpr int NT = 256;               // threads per workgroup
 constexpr int NWAVES = NT / 64;
+constexpr int kLuSink = 64;           // elements after each dense matrix that take discarded stores
 
 // myPow, kernels.cuh:52-63: repeated multiplication starting from 1.0.
 __device__ __forceinline__ double mypow(double base, int ex)

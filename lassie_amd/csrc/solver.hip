@@ -46,9 +46,19 @@ __device__ void strip_moves(double *__restrict__ M, int n, int p, int q, const i
 
 // Apply panel p (row moves, triangular solve for the U block row, trailing
 // update) to strip q (columns [q*NBC, q*NBC + ncols)), q > p.  One group.
-// The trailing update visits only the rows listed by the panel (all others
-// have zero multipliers, so a_ij = fma(-0, u, a_ij) = a_ij), unless the U
-// block holds a non-finite value, in which case every row is visited.
+// M must be followed by kLuSink writable elements.
+//
+// With 16-column panels the trailing update A22 -= L21 U12 runs on the matrix
+// cores over every row below the panel, as the oracle does: 16-row tiles,
+// computed transposed (A22^T -= U12^T L21^T) so that the lanes of a 16-lane
+// group hold consecutive rows of the column-major matrix, four 16x16x4 FP64
+// MFMAs per tile with k ascending (the instruction accumulates as a
+// sequential fma chain, the oracle's order).  A wave walks the row tiles
+// wave, wave+4, ... RT at a time with the next RT tiles' operands in flight.
+// With 8-column panels (n > 2048) the update uses vector fmas over the rows
+// listed by the panel (all others have zero multipliers, so
+// a_ij = fma(-0, u, a_ij) = a_ij), or every row when the U block holds a
+// non-finite value.
 template <int NBC>
 __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const int *perm_src,
                              const int *moved_to, const int *rows, int rowcnt, Shared &sh)
@@ -76,69 +86,156 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
             if (dst >= r0 + nbp) M[(size_t)(c0 + c) * n + dst] = sh.oldP[jj][c];
         }
     }
-    // U12 = L11^-1 * A12 (unit lower), one column per thread, k ascending
+    // U12 = L11^-1 * A12 (unit lower), one column per thread in registers,
+    // k ascending; written to M and back to LDS
+    int nonfinite = 0;
     if (tid < ncols) {
         const int c = tid;
-        for (int i = 1; i < nbp; i++) {
-            double v = sh.U12[i][c];
-            for (int kk = 0; kk < i; kk++) v = fma(-sh.L11[i][kk], sh.U12[kk][c], v);
-            sh.U12[i][c] = v;
-        }
-    }
-    __syncthreads();
-    int nonfinite = 0;
-    for (int e = tid; e < NBC * NBC; e += NT) {
-        const int kk = e / NBC, c = e % NBC;
-        if (kk < nbp && c < ncols) {
-            M[(size_t)(c0 + c) * n + r0 + kk] = sh.U12[kk][c];
-            nonfinite |= !isfinite(sh.U12[kk][c]);
-        }
-    }
-    const bool all_rows = __syncthreads_or(nonfinite) != 0;
-    // trailing update A22 -= L21 * U12, fma chain with k ascending.  A wave
-    // takes 64 rows at a time and owns CPW columns of the strip, whose U12
-    // entries stay in registers; two row blocks are in flight per iteration.
-    constexpr int CPW = NBC / NWAVES;
-    const int lane = tid & 63;
-    const int cbase = (tid >> 6) * CPW;
-    double u[NBC][CPW];
+        double u[NBC];
 #pragma unroll
-    for (int kk = 0; kk < NBC; kk++)
+        for (int kk = 0; kk < NBC; kk++) u[kk] = (kk < nbp) ? sh.U12[kk][c] : 0.0;
 #pragma unroll
-        for (int cc = 0; cc < CPW; cc++) u[kk][cc] = (kk < nbp) ? sh.U12[kk][cbase + cc] : 0.0;
-    const int nr = all_rows ? n - (r0 + nbp) : rowcnt;
-    for (int x0 = lane; x0 < nr; x0 += 128) {
-        const int x1 = x0 + 64;
-        const bool has1 = x1 < nr;
-        const int i0 = all_rows ? r0 + nbp + x0 : rows[x0];
-        const int i1 = has1 ? (all_rows ? r0 + nbp + x1 : rows[x1]) : i0;
-        double acc0[CPW], acc1[CPW], l0[NBC], l1[NBC];
+        for (int i = 1; i < NBC; i++) {
+            if (i < nbp) {
+                double v = u[i];
 #pragma unroll
-        for (int cc = 0; cc < CPW; cc++) {
-            const bool okc = cbase + cc < ncols;
-            acc0[cc] = okc ? M[(size_t)(c0 + cbase + cc) * n + i0] : 0.0;
-            acc1[cc] = (okc && has1) ? M[(size_t)(c0 + cbase + cc) * n + i1] : 0.0;
-        }
-#pragma unroll
-        for (int kk = 0; kk < NBC; kk++) {
-            l0[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + i0] : 0.0;
-            l1[kk] = (kk < nbp && has1) ? M[(size_t)(r0 + kk) * n + i1] : 0.0;
+                for (int kk = 0; kk < i; kk++) v = fma(-sh.L11[i][kk], u[kk], v);
+                u[i] = v;
+            }
         }
 #pragma unroll
         for (int kk = 0; kk < NBC; kk++) {
             if (kk < nbp) {
+                sh.U12[kk][c] = u[kk];
+                M[(size_t)(c0 + c) * n + r0 + kk] = u[kk];
+                nonfinite |= !isfinite(u[kk]);
+            }
+        }
+    }
+    const bool all_rows = __syncthreads_or(nonfinite) != 0;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if constexpr (NBC == 16) {
+        (void)all_rows;
+        (void)rows;
+        (void)rowcnt;
+        constexpr int RT = 4;
+        typedef Mfma<double>::acc Acc;
+        // global-memory view: precise memory counters keep the next tiles'
+        // loads in flight across the current MFMAs
+        typedef __attribute__((address_space(1))) double GD;
+        GD *Mg = (GD *)M;
+        const size_t last = (size_t)n * n - 1;
+        const size_t sink = (size_t)n * n;
+        const int l16 = lane & 15, lq = lane >> 4;
+        const int rb = r0 + nbp;             // first row below the panel
+        const int nt = (n - rb + 15) >> 4;   // row tiles
+        // A operand: U12^T (A[c][k] = U12[k][c]); padding columns are 0
+        double ua[4];
 #pragma unroll
-                for (int cc = 0; cc < CPW; cc++) {
-                    acc0[cc] = fma(-l0[kk], u[kk][cc], acc0[cc]);
-                    acc1[cc] = fma(-l1[kk], u[kk][cc], acc1[cc]);
+        for (int kc = 0; kc < 4; kc++) {
+            const int k = 4 * kc + lq;
+            ua[kc] = (k < nbp && l16 < ncols) ? sh.U12[k][l16] : 0.0;
+        }
+        // tile t of this wave: rows rb + 16 (wave + NWAVES t) + l16
+        auto load_group = [&](int g, Acc *dd, double (*bb)[4]) {
+#pragma unroll
+            for (int u = 0; u < RT; u++) {
+                const int r = rb + 16 * (wave + NWAVES * (g + u)) + l16;
+                const bool okr = r < n;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int c = Mfma<double>::row(lq, i);
+                    const bool ok = okr && c < ncols;
+                    const double v = Mg[ok ? (size_t)(c0 + c) * n + r : last];
+                    dd[u][i] = ok ? v : 0.0;
+                }
+#pragma unroll
+                for (int kc = 0; kc < 4; kc++) {
+                    const int k = 4 * kc + lq;
+                    const bool ok = okr && k < nbp;
+                    const double v = Mg[ok ? (size_t)(r0 + k) * n + r : last];
+                    // padding products are 0 * (-0) = -0, which leaves every value unchanged
+                    bb[u][kc] = ok ? -v : -0.0;
+                }
+            }
+        };
+        const int ntw = (nt - wave + NWAVES - 1) / NWAVES;  // tiles of this wave
+        Acc d[RT], dn[RT];
+        double b[RT][4], bn[RT][4];
+        load_group(0, d, b);
+        for (int g = 0; g < ntw; g += RT) {
+            const bool more = g + RT < ntw;
+            load_group(g + RT, dn, bn);  // unconditional: no branch, exact memory counters
+#pragma unroll
+            for (int u = 0; u < RT; u++) {
+#pragma unroll
+                for (int kc = 0; kc < 4; kc++) d[u] = Mfma<double>::run(ua[kc], b[u][kc], d[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < RT; u++) {
+                const int r = rb + 16 * (wave + NWAVES * (g + u)) + l16;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int c = Mfma<double>::row(lq, i);
+                    const bool ok = g + u < ntw && r < n && c < ncols;
+                    Mg[ok ? (size_t)(c0 + c) * n + r : sink] = d[u][i];
+                }
+            }
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < RT; u++) {
+                    d[u] = dn[u];
+#pragma unroll
+                    for (int kc = 0; kc < 4; kc++) b[u][kc] = bn[u][kc];
                 }
             }
         }
+    } else {
+        // vector update: a wave takes 64 rows at a time and owns CPW columns
+        // of the strip, whose U12 entries stay in registers; two row blocks
+        // are in flight per iteration.
+        constexpr int CPW = NBC / NWAVES;
+        const int cbase = wave * CPW;
+        double u[NBC][CPW];
 #pragma unroll
-        for (int cc = 0; cc < CPW; cc++) {
-            if (cbase + cc < ncols) {
-                M[(size_t)(c0 + cbase + cc) * n + i0] = acc0[cc];
-                if (has1) M[(size_t)(c0 + cbase + cc) * n + i1] = acc1[cc];
+        for (int kk = 0; kk < NBC; kk++)
+#pragma unroll
+            for (int cc = 0; cc < CPW; cc++) u[kk][cc] = (kk < nbp) ? sh.U12[kk][cbase + cc] : 0.0;
+        const int nr = all_rows ? n - (r0 + nbp) : rowcnt;
+        for (int x0 = lane; x0 < nr; x0 += 128) {
+            const int x1 = x0 + 64;
+            const bool has1 = x1 < nr;
+            const int i0 = all_rows ? r0 + nbp + x0 : rows[x0];
+            const int i1 = has1 ? (all_rows ? r0 + nbp + x1 : rows[x1]) : i0;
+            double acc0[CPW], acc1[CPW], l0[NBC], l1[NBC];
+#pragma unroll
+            for (int cc = 0; cc < CPW; cc++) {
+                const bool okc = cbase + cc < ncols;
+                acc0[cc] = okc ? M[(size_t)(c0 + cbase + cc) * n + i0] : 0.0;
+                acc1[cc] = (okc && has1) ? M[(size_t)(c0 + cbase + cc) * n + i1] : 0.0;
+            }
+#pragma unroll
+            for (int kk = 0; kk < NBC; kk++) {
+                l0[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + i0] : 0.0;
+                l1[kk] = (kk < nbp && has1) ? M[(size_t)(r0 + kk) * n + i1] : 0.0;
+            }
+#pragma unroll
+            for (int kk = 0; kk < NBC; kk++) {
+                if (kk < nbp) {
+#pragma unroll
+                    for (int cc = 0; cc < CPW; cc++) {
+                        acc0[cc] = fma(-l0[kk], u[kk][cc], acc0[cc]);
+                        acc1[cc] = fma(-l1[kk], u[kk][cc], acc1[cc]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < CPW; cc++) {
+                if (cbase + cc < ncols) {
+                    M[(size_t)(c0 + cbase + cc) * n + i0] = acc0[cc];
+                    if (has1) M[(size_t)(c0 + cbase + cc) * n + i1] = acc1[cc];
+                }
             }
         }
     }

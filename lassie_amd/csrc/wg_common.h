@@ -27,6 +27,31 @@ struct CtlView {
     double *y1, *ua, *ub, *k1, *k2, *k3, *k4, *k5, *w1, *yv, *x;
 };
 
+// 16x16x4 matrix-core step D = A B + C for FP64 and FP32 operands.
+template <typename T>
+struct Mfma;
+// Both take A lane l -> A[l%16][l/16] and B lane l -> B[l/16][l%16]; item i
+// of lane l holds D[row(l/16, i)][l%16], whose row order differs between the
+// two instructions (measured on gfx950).
+template <>
+struct Mfma<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc run(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lq, int i) { return lq + 4 * i; }
+};
+template <>
+struct Mfma<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc run(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lq, int i) { return 4 * lq + i; }
+};
+
 // ---------------------------------------------------------------------------
 // workgroup reductions (all threads receive the result)
 
