@@ -68,24 +68,23 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
     const int nbp = min(NBC, n - r0);
     const int c0 = q * NBC;
     const int ncols = min(NBC, n - c0);
-    // gather the permuted panel rows and the old panel-range rows; stage L11
-    for (int e = tid; e < NBC * NBC; e += NT) {
-        const int kk = e / NBC, c = e % NBC;
-        if (kk < nbp && c < ncols) {
-            sh.U12[kk][c] = M[(size_t)(c0 + c) * n + perm_src[r0 + kk]];
-            sh.oldP[kk][c] = M[(size_t)(c0 + c) * n + r0 + kk];
+    // gather the permuted panel rows and the old panel-range rows; stage L11.
+    // Thread (kk, c) also fetches where row r0+kk was displaced to.
+    static_assert(NBC * NBC <= NT, "one strip element per thread");
+    const int ek = tid / NBC, ec = tid % NBC;
+    const bool eok = tid < NBC * NBC && ek < nbp;
+    int dst = -1;
+    if (eok) {
+        if (ec < ncols) {
+            sh.U12[ek][ec] = M[(size_t)(c0 + ec) * n + perm_src[r0 + ek]];
+            sh.oldP[ek][ec] = M[(size_t)(c0 + ec) * n + r0 + ek];
+            dst = moved_to[r0 + ek];
         }
-        if (kk < nbp && c < nbp) sh.L11[kk][c] = M[(size_t)(r0 + c) * n + r0 + kk];
+        if (ec < nbp) sh.L11[ek][ec] = M[(size_t)(r0 + ec) * n + r0 + ek];
     }
     __syncthreads();
     // displaced panel-range rows go to the positions their pivots vacated
-    for (int e = tid; e < NBC * NBC; e += NT) {
-        const int jj = e / NBC, c = e % NBC;
-        if (jj < nbp && c < ncols) {
-            const int dst = moved_to[r0 + jj];
-            if (dst >= r0 + nbp) M[(size_t)(c0 + c) * n + dst] = sh.oldP[jj][c];
-        }
-    }
+    if (dst >= r0 + nbp) M[(size_t)(c0 + ec) * n + dst] = sh.oldP[ek][ec];
     // U12 = L11^-1 * A12 (unit lower), one column per thread in registers,
     // k ascending; written to M and back to LDS
     int nonfinite = 0;
@@ -112,11 +111,15 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
             }
         }
     }
-    const bool all_rows = __syncthreads_or(nonfinite) != 0;
+    // the 8-column update visits every row when U12 holds a non-finite value
+    bool all_rows = false;
+    if constexpr (NBC == 16)
+        __syncthreads();
+    else
+        all_rows = __syncthreads_or(nonfinite) != 0;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     if constexpr (NBC == 16) {
-        (void)all_rows;
         (void)rows;
         (void)rowcnt;
         constexpr int RT = 4;
@@ -257,7 +260,7 @@ __device__ void panel_fit(double *M, int n, int p, int *piv, int *perm_src, int 
             return;
         }
     }
-    panel_factor<RPT, NBC, double, false>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+    panel_factor<RPT, NBC, double, false, (NBC != 16)>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
 }
 
 // Blocked right-looking LU over all groups; strips are owned round-robin.
@@ -281,14 +284,14 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
         return inf != 0 && inf - 1 < col_end;
     };
     if (wg == 0)
-        panel_factor<RPT, NBC, double, false>(A.M, n, 0, A.piv, A.perm_src, A.moved_to, A.rows, A.rowcnt, &A.ctl->info, sh);
+        panel_fit<RPT, NBC>(A.M, n, 0, A.piv, A.perm_src, A.moved_to, A.rows, A.rowcnt, &A.ctl->info, sh);
     if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
     if (failed_before(NBC)) {
         singular = true;
         return true;
     }
     for (int p = 0; p < P; p++) {
-        const int cnt = A.rowcnt[p];
+        const int cnt = (NBC == 16) ? 0 : A.rowcnt[p];  // row lists feed the 8-column update only
         const int *rows = A.rows + (size_t)p * n;
         int q = p + 1 + ((wg - (p + 1)) % G + G) % G;
         for (; q < P; q += G) {

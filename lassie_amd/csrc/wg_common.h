@@ -341,11 +341,11 @@ __device__ __forceinline__ float readlane_d(float v, int lane)
 // are tracked as positions and applied when the panel is written back.
 // Publishes piv[k], perm_src[pos] (original row of the pivot now at pos),
 // moved_to[r0+j] (final position of the row that started at r0+j), and the
-// positions below the panel whose multipliers are not all zero (rows, rowcnt):
-// the only rows the trailing update changes.  T is the element type (double
+// positions below the panel whose multipliers are not all zero (rows, rowcnt;
+// LIST only): the only rows the trailing update changes.  T is the element type (double
 // for the FP64 method, float for the precision form); RM selects row-major
 // (else column-major) storage.
-template <int RPT, int NBC, typename T, bool RM>
+template <int RPT, int NBC, typename T, bool RM, bool LIST = true>
 __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *perm_src, int *moved_to,
                              int *rows, int *rowcnt, int *info, Shared &sh)
 {
@@ -370,13 +370,17 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
     }
     int buf = 0;
     bool stop = false;
+    int stopk = 0;
+    int pk[NBC];  // pivot rows, stored once the panel is done
     // Columns run without control flow around them (a branch per column made
-    // the compiler copy the whole register panel at every merge).  Padding
-    // columns (j >= nbp) and columns after a zero pivot compute values that
-    // are never used: they change no position, pivot or stored entry.
+    // the compiler copy the whole register panel at every merge); conditions
+    // combine with bitwise operators so they stay lane masks rather than
+    // branches.  Padding columns (j >= nbp) and columns after a zero pivot
+    // compute values that are never used: they change no position, pivot or
+    // stored entry.
 #pragma unroll
     for (int j = 0; j < NBC; j++) {
-        const bool live = j < nbp && !stop;
+        const bool live = (j < nbp) & !stop;
         const int k = r0 + j;
         double bv = -2.0;
         int bp = 0x7fffffff;
@@ -385,7 +389,7 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
         for (int q = 0; q < RPT; q++) {
             const double xv = (double)a[q][j];
             const double v = isnan(xv) ? ((pos[q] == k) ? INFINITY : -1.0) : fabs(xv);
-            const bool take = act[q] && (v > bv || (v == bv && pos[q] < bp));
+            const bool take = act[q] & ((v > bv) | ((v == bv) & (pos[q] < bp)));
             bv = take ? v : bv;
             bp = take ? pos[q] : bp;
             bq = take ? q : bq;
@@ -397,7 +401,7 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
             sh.cand_v[buf][wave] = wv;
             sh.cand_p[buf][wave] = wp;
         }
-        if (bq >= 0 && bp == wp) {
+        if ((bq >= 0) & (bp == wp)) {
 #pragma unroll
             for (int q = 0; q < RPT; q++)
                 if (q == bq) {
@@ -406,43 +410,53 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
                 }
         }
         __syncthreads();
+        // all candidates are read before any comparison (one LDS round trip)
+        double cv[NWAVES];
+        int cp[NWAVES];
+#pragma unroll
+        for (int ww = 0; ww < NWAVES; ww++) {
+            cv[ww] = sh.cand_v[buf][ww];
+            cp[ww] = sh.cand_p[buf][ww];
+        }
         int w = 0;
-        bv = sh.cand_v[buf][0];
-        bp = sh.cand_p[buf][0];
+        bv = cv[0];
+        bp = cp[0];
 #pragma unroll
         for (int ww = 1; ww < NWAVES; ww++) {
-            const double ov = sh.cand_v[buf][ww];
-            const int op = sh.cand_p[buf][ww];
-            const bool take = ov > bv || (ov == bv && op < bp);
-            bv = take ? ov : bv;
-            bp = take ? op : bp;
+            const bool take = (cv[ww] > bv) | ((cv[ww] == bv) & (cp[ww] < bp));
+            bv = take ? cv[ww] : bv;
+            bp = take ? cp[ww] : bp;
             w = take ? ww : w;
         }
         const double *prow = sh.cand_row[buf][w];
         const T ukk = (T)prow[j];
-        if (live && ukk == T(0)) {
-            if (tid == 0) st_rlx(info, k + 1);
-            stop = true;
-        }
-        const bool go = live && ukk != T(0);
-        if (go && tid == 0) piv[k] = bp;
+        const bool zero = live & (ukk == T(0));
+        stopk = (zero & !stop) ? k : stopk;
+        stop = stop | zero;
+        const bool go = live & (ukk != T(0));
+        pk[j] = go ? bp : -1;
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
             const int old = pos[q];
-            const bool isp = go && act[q] && old == bp;
-            const bool isk = go && act[q] && old == k;
+            const bool isp = go & act[q] & (old == bp);
+            const bool isk = go & act[q] & (old == k);
             pos[q] = isp ? k : (isk ? bp : old);
-            act[q] = act[q] && !isp;
+            act[q] = act[q] & !isp;
         }
         T pr[NBC];
 #pragma unroll
         for (int c = 0; c < NBC; c++) pr[c] = (c > j) ? (T)prow[c] : T(0);
+        // multipliers of all rows at once (independent divisions overlap);
         // rows already used as pivots keep their values
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
-            if (go && act[q]) {
-                const T l = a[q][j] / ukk;
-                a[q][j] = l;
+            const T l = a[q][j] / ukk;
+            a[q][j] = (go & act[q]) ? l : a[q][j];
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            if (go & act[q]) {
+                const T l = a[q][j];
 #pragma unroll
                 for (int c = 0; c < NBC; c++)
                     if (c > j) a[q][c] = fma(-l, pr[c], a[q][c]);
@@ -450,7 +464,28 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
         }
         buf ^= 1;
     }
+    if (tid == 0) {
+#pragma unroll
+        for (int j = 0; j < NBC; j++)
+            if (pk[j] >= 0) piv[r0 + j] = pk[j];
+        if (stop) st_rlx(info, stopk + 1);
+    }
     if (stop) return;
+    if (!LIST) {
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int orig = r0 + tid + NT * q;
+            if (orig < n) {
+#pragma unroll
+                for (int c = 0; c < NBC; c++)
+                    if (c < nbp) at(pos[q], r0 + c) = a[q][c];
+                if (!act[q]) perm_src[pos[q]] = orig;
+                if (orig < r0 + nbp) moved_to[orig] = pos[q];
+            }
+        }
+        __syncthreads();
+        return;
+    }
     // rows below the panel with a nonzero multiplier, listed by (q, wave, lane)
     bool nzq[RPT];
 #pragma unroll

@@ -36,7 +36,7 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
     for (int p = 0; p < P; p++) {
         const int r0 = p * NBC;
         const int nbp = min(NBC, n - r0);
-        panel_factor<RPT, NBC, T, true>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n, rowcnt + p, info,
+        panel_factor<RPT, NBC, T, true, false>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n, rowcnt + p, info,
                                         sh);
         __syncthreads();
         const int inf = ld_rlx(info);
