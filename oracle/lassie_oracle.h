@@ -1,6 +1,6 @@
 /*
- * lassie_oracle.h -- types and entry points of the oracle (CPU FP64
- * restatement of LASSIE's deterministic integrator and of its discrete
+ * lassie_oracle.h -- types and entry points of the oracle (CPU FP64 and
+ * FP32 restatements of LASSIE's deterministic integrator and of its discrete
  * adjoint).
  *
  * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and
@@ -81,6 +81,19 @@ int orc_count_bad(int S, const double *yv);
 int orc_solve_rec(const orc_model *m, const orc_opts *o, const double *tv, int nt, const int32_t *cs,
                   int ncs, double *out, double *y_final, double *progress, int64_t progress_cap,
                   orc_stats *st, orc_tape *tape);
+
+/* Single-precision loop, Simulation<float> (lassie_oracle_f32.c). */
+void orc32_rhs(const orc_model *m, const float *x, float *f);
+void orc32_jac(const orc_model *m, const float *x, float *J);
+void orc32_newton_matrix(int S, float h, float *J);
+int orc32_lu(int n, float *A, int32_t *piv);
+void orc32_lu_solve(int n, const float *LU, const int32_t *piv, float *b);
+float orc32_normsq(int n, const float *d);
+int orc32_rkf_attempt(const orc_model *m, double rtol, float dt, const float *y1, float *w1, float *w2,
+                      float *delta);
+int orc32_solve(const orc_model *m, const orc_opts *o, const double *tv, int nt, const int32_t *cs,
+                int ncs, double *out, double *y_final, double *progress, int64_t progress_cap,
+                orc_stats *st);
 
 /* Discrete adjoint (lassie_oracle_adjoint.c). */
 int orc_replay(const orc_model *m, const orc_opts *o, const orc_tape *tape, int nt, double *out_all);

@@ -94,6 +94,19 @@ def lib():
         L.orc_adjoint.argtypes = [C.POINTER(OrcModel), C.POINTER(OrcOpts), C.POINTER(OrcTape), C.c_int, _f64p,
                                   _f64p, _f64p]
         L.orc_adjoint.restype = C.c_int
+        _f32p = C.POINTER(C.c_float)
+        L.orc32_rhs.argtypes = [C.POINTER(OrcModel), _f32p, _f32p]
+        L.orc32_jac.argtypes = [C.POINTER(OrcModel), _f32p, _f32p]
+        L.orc32_newton_matrix.argtypes = [C.c_int, C.c_float, _f32p]
+        L.orc32_lu.argtypes = [C.c_int, _f32p, _i32p]
+        L.orc32_lu.restype = C.c_int
+        L.orc32_lu_solve.argtypes = [C.c_int, _f32p, _i32p, _f32p]
+        L.orc32_normsq.argtypes = [C.c_int, _f32p]
+        L.orc32_normsq.restype = C.c_float
+        L.orc32_rkf_attempt.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_float, _f32p, _f32p, _f32p, _f32p]
+        L.orc32_rkf_attempt.restype = C.c_int
+        L.orc32_solve.argtypes = L.orc_solve.argtypes
+        L.orc32_solve.restype = C.c_int
         _lib = L
     return _lib
 
@@ -200,6 +213,75 @@ def solve(m, t, cs=None, progress_cap=0, **kw):
     rc = lib().orc_solve(C.byref(b.s), C.byref(o), _p(t, _f64p), t.size, _p(cs, _i32p), cs.size,
                          _p(out, _f64p), _p(yf, _f64p),
                          _p(prog, _f64p) if progress_cap else None, progress_cap, C.byref(st))
+    d = st.as_dict()
+    n_prog = min(progress_cap, d["loop_iters"])
+    return out, d, rc, prog[:n_prog], yf
+
+
+_f32p = C.POINTER(C.c_float)
+
+
+def rhs32(m, x):
+    """generate_ode in single precision (Simulation<float>)."""
+    b = _Bound(m)
+    x = np.ascontiguousarray(x, np.float32)
+    f = np.empty(m.S, np.float32)
+    lib().orc32_rhs(C.byref(b.s), _p(x, _f32p), _p(f, _f32p))
+    return f
+
+
+def jac32(m, x):
+    b = _Bound(m)
+    x = np.ascontiguousarray(x, np.float32)
+    J = np.empty((m.S, m.S), np.float32)
+    lib().orc32_jac(C.byref(b.s), _p(x, _f32p), _p(J, _f32p))
+    return J
+
+
+def lu32(A):
+    LU = np.array(A, dtype=np.float32, order="C", copy=True)
+    n = LU.shape[0]
+    piv = np.zeros(n, np.int32)
+    info = lib().orc32_lu(n, _p(LU, _f32p), _p(piv, _i32p))
+    return LU, piv, info
+
+
+def lu_solve32(LU, piv, b):
+    x = np.array(b, dtype=np.float32, copy=True)
+    LU = np.ascontiguousarray(LU, np.float32)
+    piv = np.ascontiguousarray(piv, np.int32)
+    lib().orc32_lu_solve(LU.shape[0], _p(LU, _f32p), _p(piv, _i32p), _p(x, _f32p))
+    return x
+
+
+def normsq32(d):
+    d = np.ascontiguousarray(d, np.float32)
+    return lib().orc32_normsq(d.size, _p(d, _f32p))
+
+
+def rkf_attempt32(m, y1, dt, rtol=0.0):
+    b = _Bound(m)
+    y1 = np.ascontiguousarray(y1, np.float32)
+    w1, w2, delta = (np.empty(m.S, np.float32) for _ in range(3))
+    nok = lib().orc32_rkf_attempt(C.byref(b.s), rtol, dt, _p(y1, _f32p), _p(w1, _f32p), _p(w2, _f32p),
+                                  _p(delta, _f32p))
+    return nok, w1, w2, delta
+
+
+def solve32(m, t, cs=None, progress_cap=0, **kw):
+    """Full trajectory of the single-precision loop (the reference's `-p
+    float`); same returns as solve(), values are floats widened to double."""
+    b = _Bound(m)
+    t = np.ascontiguousarray(t, np.float64)
+    cs = np.arange(m.S, dtype=np.int32) if cs is None else np.ascontiguousarray(cs, np.int32)
+    out = np.full((t.size, cs.size), np.nan)
+    o = opts(**kw)
+    st = OrcStats()
+    prog = np.zeros(max(progress_cap, 1))
+    yf = np.zeros(m.S)
+    rc = lib().orc32_solve(C.byref(b.s), C.byref(o), _p(t, _f64p), t.size, _p(cs, _i32p), cs.size,
+                           _p(out, _f64p), _p(yf, _f64p),
+                           _p(prog, _f64p) if progress_cap else None, progress_cap, C.byref(st))
     d = st.as_dict()
     n_prog = min(progress_cap, d["loop_iters"])
     return out, d, rc, prog[:n_prog], yf
