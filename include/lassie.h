@@ -43,6 +43,9 @@ extern "C" {
 #define LASSIE_PREC_FP64 0          /* plain method: FP64 LU (cuSOLVER getrf/getrs seat) */
 #define LASSIE_PREC_FP32_REFINE 1   /* FP32 LU + FP64 iterative refinement */
 #define LASSIE_PREC_BF16_REFINE 2   /* BF16 LU + FP64 iterative refinement */
+/* the whole loop in single precision: `-p float`, Simulation<float>
+ * (simulator.cu:259-304); lassie_solve only */
+#define LASSIE_PREC_FP32_LOOP 3
 
 typedef struct lassie_model lassie_model;
 
@@ -111,7 +114,9 @@ int lassie_model_destroy(lassie_model *model);
  * (LASSIE_ERR_MAXITER) are left unchanged.  y_final[S] (may be NULL) receives the
  * state when the loop ends; progress (may be NULL, capacity progress_cap)
  * receives t/t_max at the top of every loop iteration (ProgressSimulation,
- * Simulation.cuh:660-672).  Reference: Simulation<T>::run, Simulation.cuh:230-981. */
+ * Simulation.cuh:660-672).  opts.precision LASSIE_PREC_FP64 runs Simulation<double>,
+ * LASSIE_PREC_FP32_LOOP runs Simulation<float> (values widened to double on
+ * output).  Reference: Simulation<T>::run, Simulation.cuh:230-981. */
 int lassie_solve(lassie_model *model, const lassie_opts *opts, const double *t, int32_t nt,
                  const int32_t *cs, int32_t ncs, double *out, double *y_final,
                  double *progress, int64_t progress_cap, lassie_stats *stats);

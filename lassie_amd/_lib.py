@@ -27,6 +27,7 @@ LASSIE_ERR_IO = 7
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
 PREC_BF16_REFINE = 2
+PREC_FP32_LOOP = 3
 
 _i32p = C.POINTER(C.c_int32)
 _i16p = C.POINTER(C.c_int16)

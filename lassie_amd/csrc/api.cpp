@@ -58,6 +58,11 @@ struct lassie_model {
     double *d_hk = nullptr, *d_x0 = nullptr, *d_feed = nullptr, *d_atol = nullptr;
     lassie::HRec *d_recs = nullptr;
     std::vector<lassie::HRec> recs;
+    // single-precision loop: sign * k formed in float (as the reference's
+    // Simulation<float> does), built on first use after each rate change
+    double *d_hk32 = nullptr;
+    lassie::HRec *d_recs32 = nullptr;
+    unsigned long long version32 = ~0ULL;
     double *d_vec = nullptr;  // work vectors, 12 x S
     double *d_M = nullptr;    // dense S x S Newton matrix (column-major)
     int *d_piv = nullptr, *d_perm = nullptr, *d_moved = nullptr, *d_rows = nullptr, *d_rowcnt = nullptr;
@@ -82,7 +87,7 @@ static void free_model(lassie_model *m)
                     m->d_atol, m->d_vec, m->d_M, m->d_piv, m->d_perm, m->d_moved, m->d_rows, m->d_rowcnt, m->d_ctl, m->d_gs,
                     m->d_stats, m->d_dstats, m->d_tv, m->d_out, m->d_prog, m->d_cs, m->d_recs,
                     m->d_htptr, m->d_htsp, m->d_htcoef, m->d_ltptr, m->d_ltent, m->d_lrx, m->d_k,
-                    m->d_pfull, m->d_tri, m->d_tflag};
+                    m->d_pfull, m->d_tri, m->d_tflag, m->d_hk32, m->d_recs32};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -425,13 +430,29 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
         o = *opts;
     else
         lassie_opts_default(&o);
-    if (o.precision != LASSIE_PREC_FP64)
-        return fail(LASSIE_ERR_UNSUPPORTED, "single-model solve supports LASSIE_PREC_FP64");
+    if (o.precision != LASSIE_PREC_FP64 && o.precision != LASSIE_PREC_FP32_LOOP)
+        return fail(LASSIE_ERR_UNSUPPORTED, "single-model solve supports LASSIE_PREC_FP64 and LASSIE_PREC_FP32_LOOP");
+    const int fp32 = o.precision == LASSIE_PREC_FP32_LOOP ? 1 : 0;
     if (m->S > lassie::kMaxDenseN)
         return fail(LASSIE_ERR_UNSUPPORTED, "dense Newton matrix limited to 4096 species");
     if (progress_cap < 0 || (progress_cap > 0 && !progress)) return fail(LASSIE_ERR_INVALID, "bad progress buffer");
     HIP_TRY(hipSetDevice(m->device));
     int rc;
+    const int nh = (int)m->hrx.size();
+    if (fp32 && nh && m->version32 != m->version) {
+        // (float)coef * (float)k: the float product `signum * c_vector[k_pos]`
+        std::vector<double> hk32(nh);
+        std::vector<lassie::HRec> recs32(m->recs);
+        for (int e = 0; e < nh; e++) {
+            hk32[e] = (double)((float)m->hcoef[e] * (float)m->k[m->hrx[e]]);
+            recs32[e].hk = hk32[e];
+        }
+        if (!m->d_hk32 && (rc = dev_upload(&m->d_hk32, hk32.data(), hk32.size()))) return rc;
+        if (!m->d_recs32 && (rc = dev_upload(&m->d_recs32, recs32.data(), recs32.size()))) return rc;
+        HIP_TRY(hipMemcpy(m->d_hk32, hk32.data(), sizeof(double) * nh, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(m->d_recs32, recs32.data(), sizeof(lassie::HRec) * nh, hipMemcpyHostToDevice));
+        m->version32 = m->version;
+    }
     if ((rc = ensure(&m->d_tv, &m->cap_tv, (size_t)nt))) return rc;
     if ((rc = ensure(&m->d_cs, &m->cap_cs, (size_t)ncs))) return rc;
     if ((rc = ensure(&m->d_out, &m->cap_out, (size_t)nt * ncs))) return rc;
@@ -456,7 +477,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     const bool lds_model = model_lds + static_lds <= (size_t)max_lds && !(o.reserved & 2);
     const size_t dyn = lds_model ? model_lds : sizeof(double) * (size_t)S;
     int per_cu = 0;
-    if (lassie::solver_occupancy(S, dyn, &per_cu) != 0 || per_cu < 1)
+    if (lassie::solver_occupancy(S, fp32, dyn, &per_cu) != 0 || per_cu < 1)
         return fail(LASSIE_ERR_HIP, "cannot size the cooperative launch");
     // opts.reserved bits 8..15 cap the number of groups (diagnostics)
     const int gcap = (o.reserved >> 8) & 0xff;
@@ -469,10 +490,10 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.m.lexp = m->d_lexp;
     a.m.hptr = m->d_hptr;
     a.m.hrx = m->d_hrx;
-    a.m.hk = m->d_hk;
+    a.m.hk = fp32 ? m->d_hk32 : m->d_hk;
     a.m.feed = m->d_feed;
     a.m.atol = m->d_atol;
-    a.recs = m->d_recs;
+    a.recs = fp32 ? m->d_recs32 : m->d_recs;
     a.x0 = m->d_x0;
     a.rkf_step = o.rkf_step;
     a.be_step = o.be_step;
@@ -506,6 +527,8 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.dstats_out = m->d_dstats;
     a.G = G;
     a.lds_model = lds_model ? 1 : 0;
+    a.fp32 = fp32;
+    a.yfin = m->d_vec + 6 * (size_t)S;  // past the 11 float vectors of the single-precision loop
 
     HIP_TRY(hipEventRecord(m->ev0, m->stream));
     HIP_TRY(lassie::launch_solve(a, dyn, m->stream));
@@ -516,7 +539,9 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     HIP_TRY(hipMemcpyAsync(dst, m->d_dstats, sizeof(double) * 7, hipMemcpyDeviceToHost, m->stream));
     lassie::Ctl ctl;
     HIP_TRY(hipMemcpyAsync(&ctl, m->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, m->stream));
-    if (y_final) HIP_TRY(hipMemcpyAsync(y_final, a.gvec, sizeof(double) * S, hipMemcpyDeviceToHost, m->stream));
+    if (y_final)
+        HIP_TRY(hipMemcpyAsync(y_final, fp32 ? a.yfin : a.gvec, sizeof(double) * S, hipMemcpyDeviceToHost,
+                               m->stream));
     int64_t nprog = 0;
     HIP_TRY(hipStreamSynchronize(m->stream));
     // only the stored samples (rows past them are left untouched when max_iters stops the loop)

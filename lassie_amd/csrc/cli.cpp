@@ -321,9 +321,6 @@ int main(int argc, char *argv[])
                 k[r] = k[r] * conv;
         }
     }
-    if (prec == "float")
-        std::cout << "Note: the integration runs in double precision (the FP32 whole-loop mode is not "
-                     "provided)\n";
     if (verbose > 1) std::cout << "\nThe model has " << S << " species and " << R << " reactions\n";
 
     // ---- compile and upload -----------------------------------------------------
@@ -395,6 +392,9 @@ int main(int argc, char *argv[])
     o.newton_iter = iterN;
     o.newton_tol = tollN;
     o.stiffness_tol = isStiff;
+    // -p float: Simulation<float> (simulator.cu:259-304)
+    const bool fp32 = prec == "float";
+    if (fp32) o.precision = LASSIE_PREC_FP32_LOOP;
     const int nt = (int)t.size(), ncs = (int)cs.size();
     vector<double> dinamic((size_t)nt * ncs);
     const int64_t prog_cap = 1 << 22;
@@ -443,7 +443,7 @@ int main(int argc, char *argv[])
         if (modelkind)
             for (double &v : dinamic) v = v * (volume * avo);
         for (int i = 0; i < nt; i++) {
-            fprintf(fd, "%.4e\t", t[i]);
+            fprintf(fd, "%.4e\t", fp32 ? (double)(float)t[i] : t[i]);  // t_vector holds T values
             for (int j = 0; j < ncs; j++) fprintf(fd, "%.4e\t", dinamic[(size_t)i * ncs + j]);
             fprintf(fd, "\n");
         }

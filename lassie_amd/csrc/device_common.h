@@ -44,13 +44,29 @@ struct HRec {
     int meta;
 };
 
-__device__ __forceinline__ double rhs_species_generic(const DevModel &m, const double *x, int r, double sk)
+// myPow<T> (kernels.cuh:52-63): products in the state type T, returned as
+// double.  For T = double this is mypow.
+template <typename T>
+__device__ __forceinline__ double mypow_t(T base, int ex)
+{
+    T p = 1.0;
+    while (ex > 0) {
+        p = p * base;
+        ex--;
+    }
+    return (double)p;
+}
+
+// Reactant product of reaction r (any number of reactants), as a double
+// term; sk is the coefficient product in the state type.
+template <typename T>
+__device__ __forceinline__ double rhs_species_generic(const DevModel &m, const T *x, int r, T sk)
 {
     const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
     if (a1 <= a0) return sk;
     double temp = 0.0;
     for (int j = a0; j < a1; j++) {
-        const double p = mypow(x[m.lsp[j]], m.lexp[j]);
+        const double p = mypow_t(x[m.lsp[j]], m.lexp[j]);
         if (j == a0)
             temp = temp + sk * p;
         else
@@ -59,25 +75,28 @@ __device__ __forceinline__ double rhs_species_generic(const DevModel &m, const d
     return temp;
 }
 
-// f_s of generate_ode (kernels.cuh:5-50) for one species, same operation
-// order: acc = ((0 + t_0) + t_1) + ..., t = ((0 + sk*x_a^ea) * x_b^eb).
-__device__ __forceinline__ double rhs_species(const DevModel &m, const HRec *recs, const int *hptr,
-                                              const double *x, int s)
+// f_s of generate_ode<T> (kernels.cuh:5-50) for one species, same operation
+// order: acc = ((0 + t_0) + t_1) + ..., t = ((0 + sk*x_a^ea) * x_b^eb).  The
+// term t is a double; the output, of the state type T, is rounded after
+// every term (y1[gid] = y1[gid] + temp).  rc.hk holds sign * k formed in T.
+template <typename T>
+__device__ __forceinline__ T rhs_species(const DevModel &m, const HRec *recs, const int *hptr, const T *x, int s)
 {
-    double acc = 0.0;
+    T acc = 0.0;
     const int e1 = hptr[s + 1];
     for (int e = hptr[s]; e < e1; e++) {
         const HRec rc = recs[e];
         const int kind = rc.meta >> 16;
+        const T hk = (T)rc.hk;
         if (kind == 0) {
-            acc = acc + rc.hk;
+            acc = acc + hk;
         } else if (kind <= 2) {
             double temp = 0.0;
-            temp = temp + rc.hk * mypow(x[rc.sp01 & 0xffff], rc.meta & 0xff);
-            if (kind == 2) temp = temp * mypow(x[(unsigned)rc.sp01 >> 16], (rc.meta >> 8) & 0xff);
-            acc = acc + temp;
+            temp = temp + hk * mypow_t(x[rc.sp01 & 0xffff], rc.meta & 0xff);
+            if (kind == 2) temp = temp * mypow_t(x[(unsigned)rc.sp01 >> 16], (rc.meta >> 8) & 0xff);
+            acc = (T)(acc + temp);
         } else {
-            acc = acc + rhs_species_generic(m, x, rc.sp01, rc.hk);
+            acc = (T)(acc + rhs_species_generic(m, x, rc.sp01, hk));
         }
     }
     return acc;

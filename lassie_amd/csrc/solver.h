@@ -45,13 +45,14 @@ struct SolveArgs {
     double *out;
     double *progress;
     long long progress_cap;
-    // state vectors in global memory (S each): y1 u u2 k1..k5 w1 yv x
+    // state vectors in global memory (S each): y1 u u2 k1..k5 w1 yv x, of the
+    // state type (double, or float when fp32)
     double *gvec;
-    double *M;
+    double *M;             // dense S x S Newton matrix, of the state type
     int *piv, *perm_src, *moved_to;
     int *rows, *rowcnt;    // per panel: positions with nonzero multipliers
     int *perm_full;        // final row order of the factors: row i of L U is row perm_full[i] of M
-    double *tri;           // 3 x S: permuted right-hand side, forward result, solution
+    double *tri;           // 3 x S of the state type: permuted right-hand side, forward result, solution
     unsigned int *tflag;   // 2 x blocks x kFlagStride: forward / backward block stamps
     Ctl *ctl;
     GridSync *gs;
@@ -59,10 +60,12 @@ struct SolveArgs {
     double *dstats_out;    // t, dt, newdt, phase ms x4
     int G;                 // workgroups in the cooperative launch
     int lds_model;         // 1: control group keeps model + vectors in LDS
+    int fp32;              // 1: single-precision loop (state, time, M and LU in float)
+    double *yfin;          // fp32 only: final state widened to double (S)
 };
 
 hipError_t launch_solve(const SolveArgs &args, size_t dyn_lds, hipStream_t stream);
-int solver_occupancy(int n, size_t dyn_lds, int *per_cu);
+int solver_occupancy(int n, int fp32, size_t dyn_lds, int *per_cu);
 int dense_panel_width(int n);
 size_t lds_model_bytes(int S, int nnz_h);
 

@@ -8,6 +8,12 @@
 // Every floating-point operation follows the order written in the reference
 // kernels (kernels.cuh) with no contraction (built with -ffp-contract=off),
 // so the trajectory is bit-identical to the CPU restatement used as oracle.
+//
+// T is the state type: double for the reference's default, float for its
+// `-p float` mode (Simulation<float>, simulator.cu:259-304), in which the
+// state vectors, t, dt, the Newton matrix and its LU are float while the
+// expressions with double literals and the per-term RHS/Jacobian
+// accumulators stay double, as the reference's kernels are written.
 #include "device_common.h"
 #include "solver.h"
 #include "wg_common.h"
@@ -16,9 +22,9 @@ namespace lassie {
 
 // Row moves of panel p applied to a strip left of it (q < p), so that every
 // L column ends in the final row order (LAPACK's dlaswp on the left part).
-template <int NBC>
-__device__ void strip_moves(double *__restrict__ M, int n, int p, int q, const int *perm_src,
-                            const int *moved_to, Shared &sh)
+template <int NBC, typename T>
+__device__ void strip_moves(T *__restrict__ M, int n, int p, int q, const int *perm_src, const int *moved_to,
+                            Shared &sh)
 {
     const int tid = threadIdx.x;
     const int r0 = p * NBC;
@@ -48,20 +54,20 @@ __device__ void strip_moves(double *__restrict__ M, int n, int p, int q, const i
 // update) to strip q (columns [q*NBC, q*NBC + ncols)), q > p.  One group.
 // M must be followed by kLuSink writable elements.
 //
-// With 16-column panels the trailing update A22 -= L21 U12 runs on the matrix
+// With 16-column FP64 panels the trailing update A22 -= L21 U12 runs on the matrix
 // cores over every row below the panel, as the oracle does: 16-row tiles,
 // computed transposed (A22^T -= U12^T L21^T) so that the lanes of a 16-lane
 // group hold consecutive rows of the column-major matrix, four 16x16x4 FP64
 // MFMAs per tile with k ascending (the instruction accumulates as a
 // sequential fma chain, the oracle's order).  A wave walks the row tiles
 // wave, wave+4, ... RT at a time with the next RT tiles' operands in flight.
-// With 8-column panels (n > 2048) the update uses vector fmas over the rows
+// Otherwise (8-column panels, n > 2048, or float) the update uses vector fmas over the rows
 // listed by the panel (all others have zero multipliers, so
 // a_ij = fma(-0, u, a_ij) = a_ij), or every row when the U block holds a
 // non-finite value.
-template <int NBC>
-__device__ void strip_update(double *__restrict__ M, int n, int p, int q, const int *perm_src,
-                             const int *moved_to, const int *rows, int rowcnt, Shared &sh)
+template <int NBC, typename T>
+__device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *perm_src, const int *moved_to,
+                             const int *rows, int rowcnt, Shared &sh)
 {
     const int tid = threadIdx.x;
     const int r0 = p * NBC;
@@ -90,15 +96,15 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
     int nonfinite = 0;
     if (tid < ncols) {
         const int c = tid;
-        double u[NBC];
+        T u[NBC];
 #pragma unroll
-        for (int kk = 0; kk < NBC; kk++) u[kk] = (kk < nbp) ? sh.U12[kk][c] : 0.0;
+        for (int kk = 0; kk < NBC; kk++) u[kk] = (kk < nbp) ? (T)sh.U12[kk][c] : (T)0.0;
 #pragma unroll
         for (int i = 1; i < NBC; i++) {
             if (i < nbp) {
-                double v = u[i];
+                T v = u[i];
 #pragma unroll
-                for (int kk = 0; kk < i; kk++) v = fma(-sh.L11[i][kk], u[kk], v);
+                for (int kk = 0; kk < i; kk++) v = fma(-(T)sh.L11[i][kk], u[kk], v);
                 u[i] = v;
             }
         }
@@ -112,14 +118,15 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
         }
     }
     // the 8-column update visits every row when U12 holds a non-finite value
+    constexpr bool kMfma = NBC == 16 && sizeof(T) == 8;
     bool all_rows = false;
-    if constexpr (NBC == 16)
+    if constexpr (kMfma)
         __syncthreads();
     else
         all_rows = __syncthreads_or(nonfinite) != 0;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    if constexpr (NBC == 16) {
+    if constexpr (kMfma) {
         (void)rows;
         (void)rowcnt;
         constexpr int RT = 4;
@@ -200,28 +207,28 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
         // are in flight per iteration.
         constexpr int CPW = NBC / NWAVES;
         const int cbase = wave * CPW;
-        double u[NBC][CPW];
+        T u[NBC][CPW];
 #pragma unroll
         for (int kk = 0; kk < NBC; kk++)
 #pragma unroll
-            for (int cc = 0; cc < CPW; cc++) u[kk][cc] = (kk < nbp) ? sh.U12[kk][cbase + cc] : 0.0;
+            for (int cc = 0; cc < CPW; cc++) u[kk][cc] = (kk < nbp) ? (T)sh.U12[kk][cbase + cc] : (T)0.0;
         const int nr = all_rows ? n - (r0 + nbp) : rowcnt;
         for (int x0 = lane; x0 < nr; x0 += 128) {
             const int x1 = x0 + 64;
             const bool has1 = x1 < nr;
             const int i0 = all_rows ? r0 + nbp + x0 : rows[x0];
             const int i1 = has1 ? (all_rows ? r0 + nbp + x1 : rows[x1]) : i0;
-            double acc0[CPW], acc1[CPW], l0[NBC], l1[NBC];
+            T acc0[CPW], acc1[CPW], l0[NBC], l1[NBC];
 #pragma unroll
             for (int cc = 0; cc < CPW; cc++) {
                 const bool okc = cbase + cc < ncols;
-                acc0[cc] = okc ? M[(size_t)(c0 + cbase + cc) * n + i0] : 0.0;
-                acc1[cc] = (okc && has1) ? M[(size_t)(c0 + cbase + cc) * n + i1] : 0.0;
+                acc0[cc] = okc ? M[(size_t)(c0 + cbase + cc) * n + i0] : (T)0.0;
+                acc1[cc] = (okc && has1) ? M[(size_t)(c0 + cbase + cc) * n + i1] : (T)0.0;
             }
 #pragma unroll
             for (int kk = 0; kk < NBC; kk++) {
-                l0[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + i0] : 0.0;
-                l1[kk] = (kk < nbp && has1) ? M[(size_t)(r0 + kk) * n + i1] : 0.0;
+                l0[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + i0] : (T)0.0;
+                l1[kk] = (kk < nbp && has1) ? M[(size_t)(r0 + kk) * n + i1] : (T)0.0;
             }
 #pragma unroll
             for (int kk = 0; kk < NBC; kk++) {
@@ -250,17 +257,19 @@ __device__ void strip_update(double *__restrict__ M, int n, int p, int q, const 
 // processed without per-row control flow, so a smaller RPT is less work.  The
 // rows each thread holds, and so every result, are the same for any RPT that
 // covers them.
-template <int RPT, int NBC>
-__device__ void panel_fit(double *M, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
-                          int *rowcnt, int *info, Shared &sh)
+template <int RPT, int NBC, typename T>
+__device__ void panel_fit(T *M, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
+                          int *info, Shared &sh)
 {
     if constexpr (RPT > 1) {
         if (n - p * NBC <= NT * (RPT / 2)) {
-            panel_fit<RPT / 2, NBC>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+            panel_fit<RPT / 2, NBC, T>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
             return;
         }
     }
-    panel_factor<RPT, NBC, double, false, (NBC != 16)>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+    // row lists feed the vector strip update only
+    constexpr bool kList = !(NBC == 16 && sizeof(T) == 8);
+    panel_factor<RPT, NBC, T, false, kList>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
 }
 
 // Blocked right-looking LU over all groups; strips are owned round-robin.
@@ -268,10 +277,11 @@ __device__ void panel_fit(double *M, int n, int p, int *piv, int *perm_src, int 
 // (look-ahead), so each panel costs one grid barrier; left strips receive the
 // panel's row moves.  On return M holds L\U in final row order.  Returns
 // false on abort.
-template <int RPT, int NBC>
+template <int RPT, int NBC, typename T>
 __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned int &gen_local,
                                bool &singular)
 {
+    T *M = reinterpret_cast<T *>(A.M);
     const int n = A.m.S;
     const int P = (n + NBC - 1) / NBC;
     const int G = A.G;
@@ -284,26 +294,26 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
         return inf != 0 && inf - 1 < col_end;
     };
     if (wg == 0)
-        panel_fit<RPT, NBC>(A.M, n, 0, A.piv, A.perm_src, A.moved_to, A.rows, A.rowcnt, &A.ctl->info, sh);
+        panel_fit<RPT, NBC, T>(M, n, 0, A.piv, A.perm_src, A.moved_to, A.rows, A.rowcnt, &A.ctl->info, sh);
     if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
     if (failed_before(NBC)) {
         singular = true;
         return true;
     }
     for (int p = 0; p < P; p++) {
-        const int cnt = (NBC == 16) ? 0 : A.rowcnt[p];  // row lists feed the 8-column update only
+        const int cnt = (NBC == 16 && sizeof(T) == 8) ? 0 : A.rowcnt[p];  // row lists feed the vector update only
         const int *rows = A.rows + (size_t)p * n;
         int q = p + 1 + ((wg - (p + 1)) % G + G) % G;
         for (; q < P; q += G) {
-            strip_update<NBC>(A.M, n, p, q, A.perm_src, A.moved_to, rows, cnt, sh);
+            strip_update<NBC>(M, n, p, q, A.perm_src, A.moved_to, rows, cnt, sh);
             if (q == p + 1) {
                 __syncthreads();
-                panel_fit<RPT, NBC>(A.M, n, p + 1, A.piv, A.perm_src, A.moved_to, A.rows + (size_t)(p + 1) * n,
-                                    A.rowcnt + p + 1, &A.ctl->info, sh);
+                panel_fit<RPT, NBC, T>(M, n, p + 1, A.piv, A.perm_src, A.moved_to, A.rows + (size_t)(p + 1) * n,
+                                       A.rowcnt + p + 1, &A.ctl->info, sh);
                 __syncthreads();
             }
         }
-        for (int ql = wg; ql < p; ql += G) strip_moves<NBC>(A.M, n, p, ql, A.perm_src, A.moved_to, sh);
+        for (int ql = wg; ql < p; ql += G) strip_moves<NBC>(M, n, p, ql, A.perm_src, A.moved_to, sh);
         if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
         if (p + 1 < P && failed_before((p + 2) * NBC)) {
             singular = true;
@@ -348,6 +358,7 @@ __device__ __forceinline__ void publish_stamp(unsigned int *flag, unsigned int v
 }
 
 // L y = b for block j (wave 0 of its owner): b = tri[0..S), y -> tri[S..2S).
+template <typename T>
 __device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp)
 {
     constexpr int TB = kTriBlock;
@@ -357,27 +368,27 @@ __device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp)
     const int nb = min(TB, n - R0);
     const int i = R0 + lane;
     const bool row = lane < nb;
-    const double *M = A.M;
-    const double *bp = A.tri;
-    double *yb = A.tri + n;
+    const T *M = reinterpret_cast<const T *>(A.M);
+    const T *bp = reinterpret_cast<const T *>(A.tri);
+    T *yb = reinterpret_cast<T *>(A.tri) + n;
     unsigned int *fwd = A.tflag;
     // the diagonal block's L entries first: the chain below must not wait on memory
-    double d[TB];
+    T d[TB];
 #pragma unroll
-    for (int k = 0; k < TB; k++) d[k] = (k < lane && row) ? M[(size_t)(R0 + k) * n + i] : 0.0;
-    double acc = row ? bp[i] : 0.0;
+    for (int k = 0; k < TB; k++) d[k] = (k < lane && row) ? M[(size_t)(R0 + k) * n + i] : (T)0.0;
+    T acc = row ? bp[i] : (T)0.0;
     for (int B = 0; B < j; B++) {
         const int c0 = B * TB;
-        double l[16], ln[16];
+        T l[16], ln[16];
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) l[kk] = row ? M[(size_t)(c0 + kk) * n + i] : 0.0;
+        for (int kk = 0; kk < 16; kk++) l[kk] = row ? M[(size_t)(c0 + kk) * n + i] : (T)0.0;
         if (!wait_stamp(fwd + (size_t)B * kFlagStride, stamp, &A.ctl->abort)) return false;
-        const double yB = yb[c0 + lane];
+        const T yB = yb[c0 + lane];
 #pragma unroll
         for (int sb = 0; sb < TB / 16; sb++) {
             if (sb + 1 < TB / 16) {
 #pragma unroll
-                for (int kk = 0; kk < 16; kk++) ln[kk] = row ? M[(size_t)(c0 + (sb + 1) * 16 + kk) * n + i] : 0.0;
+                for (int kk = 0; kk < 16; kk++) ln[kk] = row ? M[(size_t)(c0 + (sb + 1) * 16 + kk) * n + i] : (T)0.0;
             }
 #pragma unroll
             for (int kk = 0; kk < 16; kk++) acc = fma(-l[kk], readlane_d(yB, sb * 16 + kk), acc);
@@ -391,7 +402,7 @@ __device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp)
 #pragma unroll
     for (int k = 0; k < TB; k++) {
         if (k < nb) {
-            const double yk = readlane_d(acc, k);
+            const T yk = readlane_d(acc, k);
             if (lane > k && row) acc = fma(-d[k], yk, acc);
         }
     }
@@ -401,6 +412,7 @@ __device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp)
 }
 
 // U x = y for block j: y = tri[S..2S), x -> tri[2S..3S).
+template <typename T>
 __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned int stamp)
 {
     constexpr int TB = kTriBlock;
@@ -410,33 +422,33 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
     const int nb = min(TB, n - R0);
     const int i = R0 + lane;
     const bool row = lane < nb;
-    const double *M = A.M;
-    const double *yb = A.tri + n;
-    double *xb = A.tri + 2 * (size_t)n;
+    const T *M = reinterpret_cast<const T *>(A.M);
+    const T *yb = reinterpret_cast<const T *>(A.tri) + n;
+    T *xb = reinterpret_cast<T *>(A.tri) + 2 * (size_t)n;
     unsigned int *bwd = A.tflag + (size_t)nblk * kFlagStride;
     // the diagonal block's U entries (with the diagonal) first
-    double d[TB];
+    T d[TB];
 #pragma unroll
-    for (int k = 0; k < TB; k++) d[k] = (k >= lane && k < nb && row) ? M[(size_t)(R0 + k) * n + i] : 1.0;
-    double acc = row ? yb[i] : 0.0;
+    for (int k = 0; k < TB; k++) d[k] = (k >= lane && k < nb && row) ? M[(size_t)(R0 + k) * n + i] : (T)1.0;
+    T acc = row ? yb[i] : (T)0.0;
     for (int B = nblk - 1; B > j; B--) {
         const int c0 = B * TB;
         const int nbB = min(TB, n - c0);
-        double u[16], un[16];
+        T u[16], un[16];
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
             const int k = TB - 16 + kk;
-            u[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : 0.0;
+            u[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : (T)0.0;
         }
         if (!wait_stamp(bwd + (size_t)B * kFlagStride, stamp, &A.ctl->abort)) return false;
-        const double xB = (lane < nbB) ? xb[c0 + lane] : 0.0;
+        const T xB = (lane < nbB) ? xb[c0 + lane] : (T)0.0;
 #pragma unroll
         for (int sb = TB / 16 - 1; sb >= 0; sb--) {
             if (sb > 0) {
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) {
                     const int k = (sb - 1) * 16 + kk;
-                    un[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : 0.0;
+                    un[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : (T)0.0;
                 }
             }
 #pragma unroll
@@ -455,7 +467,7 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
     for (int k = TB - 1; k >= 0; k--) {
         if (k < nb) {
             if (lane == k) acc = acc / d[k];
-            const double xk = readlane_d(acc, k);
+            const T xk = readlane_d(acc, k);
             if (lane < k) acc = fma(-d[k], xk, acc);
         }
     }
@@ -465,7 +477,7 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
 }
 
 // ---------------------------------------------------------------------------
-template <int RPT, int NBC>
+template <int RPT, int NBC, typename T>
 __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
 {
     extern __shared__ __align__(16) unsigned char dyn[];
@@ -479,14 +491,14 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     Ctl *ctl = A.ctl;
 
     // ---- control-group views ---------------------------------------------
-    CtlView v;
-    double *gv = A.gvec;
+    CtlViewT<T> v;
+    T *gv = reinterpret_cast<T *>(A.gvec);
     const size_t Sz = (size_t)S;
     if (A.lds_model) {
         const int nh = A.m.hptr[S];
         HRec *lrec = reinterpret_cast<HRec *>(dyn);
         int *lhptr = reinterpret_cast<int *>(dyn + sizeof(HRec) * (size_t)nh);
-        double *lv = reinterpret_cast<double *>(dyn + ((sizeof(HRec) * (size_t)nh + 4 * (Sz + 1) + 15) & ~size_t(15)));
+        T *lv = reinterpret_cast<T *>(dyn + ((sizeof(HRec) * (size_t)nh + 4 * (Sz + 1) + 15) & ~size_t(15)));
         v.recs = lrec;
         v.hptr = lhptr;
         v.y1 = lv;
@@ -517,14 +529,14 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         v.k5 = gv + 7 * Sz;
         v.w1 = gv + 8 * Sz;
         v.yv = gv + 9 * Sz;
-        v.x = reinterpret_cast<double *>(dyn);  // the solve vector lives in LDS
+        v.x = reinterpret_cast<T *>(dyn);  // the solve vector lives in LDS
     }
-    double *gyv = gv + 9 * Sz;  // y_v as the other groups read it
-    double *gy1 = gv;
+    T *gyv = gv + 9 * Sz;  // y_v as the other groups read it
+    T *gy1 = gv;
 
     // state of the control loop, held identically by every thread of group 0
-    double t = 0.0, t1 = A.tv[0], dt = A.rkf_step, newdt = A.be_step;
-    const double t_max = A.tv[A.nt - 1];
+    T t = 0.0, t1 = (T)A.tv[0], dt = (T)A.rkf_step, newdt = (T)A.be_step;
+    const double t_max = (T)A.tv[A.nt - 1];
     int i_sample = 0;
     long long nrej = 0;
     long long st_loop = 0, st_att = 0, st_acc = 0, st_bdf = 0, st_newton = 0, st_lufail = 0, st_rep = 0,
@@ -547,7 +559,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         if (wg == 0) {
             tmark = wall_clock64();
             int mode = MODE_CONTINUE;
-            double hb = 0.0;
+            T hb = 0.0;
             if (A.max_iters > 0 && st_loop >= A.max_iters) {
                 mode = MODE_DONE;
                 rc = 1;
@@ -562,30 +574,30 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                     if (i_sample == A.nt)
                         finished = true;
                     else
-                        t1 = A.tv[i_sample];
+                        t1 = (T)A.tv[i_sample];
                 }
                 if (finished) {
                     mode = MODE_DONE;
                 } else {
                     st_att++;
-                    double mind;
+                    double mind;  // the smallest step factor (a T value)
                     const int nok = rkf_attempt_wg(A.m, A.rtol, v, dt, &mind, sh);
                     if (nok == S) {
                         nrej = 0;
                         st_acc++;
                         for (int s = tid; s < S; s += NT) v.y1[s] = v.w1[s];
                         __syncthreads();
-                        const double d_temp = dt * mind;
+                        const T d_temp = dt * (T)mind;
                         if ((t + d_temp) > t1)
                             dt = fabs(t1 - t);
                         else
                             dt = d_temp;
                         t = t + dt;
                     } else {
-                        const double dt_old = dt;
-                        dt = dt * mind;
+                        const T dt_old = dt;
+                        dt = dt * (T)mind;
                         nrej++;
-                        if (dt < A.stiffness_tol || isnan(dt) || isinf(dt) || fabs(dt_old - dt) < 1e-3 ||
+                        if (dt < (T)A.stiffness_tol || isnan(dt) || isinf(dt) || fabs(dt_old - dt) < 1e-3 ||
                             (A.max_rkf_retries > 0 && nrej >= A.max_rkf_retries)) {
                             nrej = 0;
                             st_bdf++;
@@ -595,10 +607,10 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                             mode = (A.newton_iter > 0) ? MODE_BDF : MODE_BDF_NOOP;
                             // y_v = y1; Newton iteration 0 residual at y_v
                             for (int s = tid; s < S; s += NT) {
-                                const double yv = v.y1[s];
+                                const T yv = v.y1[s];
                                 v.yv[s] = yv;
                                 gyv[s] = yv;
-                                const double f = rhs_species(A.m, v.recs, v.hptr, v.y1, s);
+                                const T f = rhs_species(A.m, v.recs, v.hptr, (const T *)v.y1, s);
                                 v.x[s] = -(yv - v.y1[s] - hb * f);
                             }
                             __syncthreads();
@@ -623,25 +635,30 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         const int mode = ld_rlx(&ctl->mode[parity]);
         if (mode == MODE_DONE) break;
         if (mode == MODE_CONTINUE) continue;
-        const double h = ld_rlx_d(&ctl->h[parity]);
+        const T h = (T)ld_rlx_d(&ctl->h[parity]);
 
         bool singular = false;
         if (mode == MODE_BDF) {
             // ---- dense M = I - hJ(y_v), all groups -------------------------------
             {
+                // zero fill: 16-byte stores, then the elements left over
                 const size_t nn = (size_t)n * n;
                 const size_t stride = (size_t)G * NT;
-                double2 *M2 = reinterpret_cast<double2 *>(A.M);
-                const double2 z = make_double2(0.0, 0.0);
-                for (size_t e = ((size_t)wg * NT + tid); e < nn / 2; e += stride) M2[e] = z;
-                if ((nn & 1) && wg == 0 && tid == 0) A.M[nn - 1] = 0.0;
+                const size_t n16 = nn * sizeof(T) / 16;
+                uint4 *M16 = reinterpret_cast<uint4 *>(A.M);
+                const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+                for (size_t e = ((size_t)wg * NT + tid); e < n16; e += stride) M16[e] = z;
+                T *Mt = reinterpret_cast<T *>(A.M);
+                if (wg == 0)
+                    for (size_t e = n16 * 16 / sizeof(T) + tid; e < nn; e += NT) Mt[e] = (T)0.0;
             }
             if (G > 1) {
                 if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
             } else {
                 __syncthreads();
             }
-            for (int r = wg * NT + tid; r < n; r += G * NT) jac_row<false>(A.m, gyv, A.M, n, r, h);
+            for (int r = wg * NT + tid; r < n; r += G * NT)
+                jac_row<false>(A.m, (const T *)gyv, reinterpret_cast<T *>(A.M), n, r, h);
             if (G > 1) {
                 if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
             } else {
@@ -652,7 +669,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 ph[1] += now - tmark;
                 tmark = now;
             }
-            if (!lu_factor_grid<RPT, NBC>(A, wg, sh, gen_local, singular)) return;
+            if (!lu_factor_grid<RPT, NBC, T>(A, wg, sh, gen_local, singular)) return;
             if (wg == 0) {
                 const unsigned long long now = wall_clock64();
                 ph[2] += now - tmark;
@@ -694,14 +711,14 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             if (wg < nblk) {
                 __shared__ int s_cont;
                 int itN = 0;
-                double errN = 1000;
+                T errN = 1000;
                 while (true) {
                     tri_stamp++;
                     if (wg == 0) {
-                        int cont = (itN < A.newton_iter && errN > A.newton_tol) ? 1 : 0;
+                        int cont = (itN < A.newton_iter && errN > (T)A.newton_tol) ? 1 : 0;
                         if (cont && itN > 0) {
                             for (int s = tid; s < S; s += NT) {
-                                const double f = rhs_species(A.m, v.recs, v.hptr, v.yv, s);
+                                const T f = rhs_species(A.m, v.recs, v.hptr, (const T *)v.yv, s);
                                 v.x[s] = -(v.yv[s] - v.y1[s] - h * f);
                             }
                             __syncthreads();
@@ -716,7 +733,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                             cont = 0;
                         }
                         if (cont)
-                            for (int s = tid; s < S; s += NT) A.tri[s] = v.x[A.perm_full[s]];
+                            for (int s = tid; s < S; s += NT) reinterpret_cast<T *>(A.tri)[s] = v.x[A.perm_full[s]];
                         __syncthreads();
                         if (tid == 0) {
                             __threadfence();
@@ -738,30 +755,31 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                     __shared__ int s_ok;
                     if (tid < 64) {
                         bool ok = true;
-                        for (int j = wg; j < nblk && ok; j += G) ok = tri_forward_block(A, j, tri_stamp);
+                        for (int j = wg; j < nblk && ok; j += G) ok = tri_forward_block<T>(A, j, tri_stamp);
                         int jl = wg;
                         while (jl + G < nblk) jl += G;
-                        for (int j = jl; j >= 0 && ok; j -= G) ok = tri_backward_block(A, j, nblk, tri_stamp);
+                        for (int j = jl; j >= 0 && ok; j -= G) ok = tri_backward_block<T>(A, j, nblk, tri_stamp);
                         if (tid == 0) s_ok = ok ? 1 : 0;
                     }
                     __syncthreads();
                     if (!s_ok) return;
                     if (wg == 0) {
                         __threadfence();
-                        const double *xs = A.tri + 2 * (size_t)n;
-                        double part = 0.0;
+                        const T *xs = reinterpret_cast<const T *>(A.tri) + 2 * (size_t)n;
+                        T part = 0.0;
                         for (int s = tid; s < S; s += NT) {
-                            const double yv1 = v.yv[s];
-                            double yv;
-                            if (A.m.feed[s] == 0)
+                            const T yv1 = v.yv[s];
+                            const T fv = (T)A.m.feed[s];
+                            T yv;
+                            if (fv == 0)
                                 yv = yv1 + xs[s];
                             else
-                                yv = A.m.feed[s];
+                                yv = fv;
                             v.yv[s] = yv;
-                            const double d = yv - yv1;
-                            part = part + (double)(float)(d * d);
+                            const T d = yv - yv1;
+                            part = part + norm_square(d);
                         }
-                        errN = sqrt(wg_tree_sum(part, sh));
+                        errN = sqrt_t(wg_tree_sum(part, sh));
                         itN = itN + 1;
                     }
                 }
@@ -779,7 +797,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                     newdt = h * 0.1;
                     st_rep++;
                 } else if (A.bdf_regrow) {
-                    newdt = fmin(newdt * 2.0, A.be_step);
+                    newdt = (T)fmin((double)newdt * 2.0, (double)(T)A.be_step);
                 }
             }
             __syncthreads();
@@ -792,6 +810,8 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     if (wg == 0) {
         if (A.lds_model)
             for (int s = tid; s < S; s += NT) gy1[s] = v.y1[s];
+        if (sizeof(T) == 4)
+            for (int s = tid; s < S; s += NT) A.yfin[s] = v.y1[s];
         if (tid == 0) {
             long long *o = A.stats_out;
             o[0] = st_loop;
@@ -821,25 +841,26 @@ size_t lds_model_bytes(int S, int nnz_h)
 }
 
 template <int RPT, int NBC>
-static const void *kernel_ptr()
+static const void *kernel_ptr(int fp32)
 {
-    return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC>);
+    return fp32 ? reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, float>)
+                : reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double>);
 }
 
-static const void *pick_kernel(int n)
+static const void *pick_kernel(int n, int fp32)
 {
-    if (n <= 256) return kernel_ptr<1, 16>();
-    if (n <= 512) return kernel_ptr<2, 16>();
-    if (n <= 1024) return kernel_ptr<4, 16>();
-    if (n <= 2048) return kernel_ptr<8, 16>();
-    return kernel_ptr<16, 8>();
+    if (n <= 256) return kernel_ptr<1, 16>(fp32);
+    if (n <= 512) return kernel_ptr<2, 16>(fp32);
+    if (n <= 1024) return kernel_ptr<4, 16>(fp32);
+    if (n <= 2048) return kernel_ptr<8, 16>(fp32);
+    return kernel_ptr<16, 8>(fp32);
 }
 
 int dense_panel_width(int n) { return n <= 2048 ? 16 : 8; }
 
-int solver_occupancy(int n, size_t dyn_lds, int *per_cu)
+int solver_occupancy(int n, int fp32, size_t dyn_lds, int *per_cu)
 {
-    const void *k = pick_kernel(n);
+    const void *k = pick_kernel(n, fp32);
     if (dyn_lds > 64 * 1024 &&
         hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) != hipSuccess)
         return -1;
@@ -851,7 +872,7 @@ hipError_t launch_solve(const SolveArgs &args, size_t dyn_lds, hipStream_t strea
 {
     SolveArgs a = args;
     void *params[] = {&a};
-    return hipLaunchCooperativeKernel(pick_kernel(a.m.S), dim3(a.G), dim3(NT), params,
+    return hipLaunchCooperativeKernel(pick_kernel(a.m.S, a.fp32), dim3(a.G), dim3(NT), params,
                                       (unsigned int)dyn_lds, stream);
 }
 

@@ -21,11 +21,18 @@ struct Shared {
 };
 
 // Pointers used by the control group: LDS when the model fits, else global.
-struct CtlView {
+// T is the state type (double; float for the single-precision loop).
+template <typename T>
+struct CtlViewT {
     const HRec *recs;
     const int *hptr;
-    double *y1, *ua, *ub, *k1, *k2, *k3, *k4, *k5, *w1, *yv, *x;
+    T *y1, *ua, *ub, *k1, *k2, *k3, *k4, *k5, *w1, *yv, *x;
 };
+typedef CtlViewT<double> CtlView;
+
+// square root in the state type (correctly rounded for double and float)
+__device__ __forceinline__ double sqrt_t(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_t(float x) { return sqrtf(x); }
 
 // 16x16x4 matrix-core step D = A B + C for FP64 and FP32 operands.
 template <typename T>
@@ -56,17 +63,19 @@ struct Mfma<float> {
 // workgroup reductions (all threads receive the result)
 
 // sum in the fixed "tid strided, then halving tree" order used for the Newton
-// norm (Simulation.cuh:883-888 with a float-returning square).
-__device__ double wg_tree_sum(double v, Shared &sh)
+// norm (Simulation.cuh:883-888 with a float-returning square), in the
+// arithmetic of T (thrust::plus<T>).
+template <typename T>
+__device__ T wg_tree_sum(T v, Shared &sh)
 {
     const int tid = threadIdx.x;
     sh.red[tid] = v;
     __syncthreads();
     for (int stride = NT / 2; stride > 0; stride >>= 1) {
-        if (tid < stride) sh.red[tid] = sh.red[tid] + sh.red[tid + stride];
+        if (tid < stride) sh.red[tid] = (T)sh.red[tid] + (T)sh.red[tid + stride];
         __syncthreads();
     }
-    const double r = sh.red[0];
+    const T r = (T)sh.red[0];
     __syncthreads();
     return r;
 }
@@ -103,60 +112,67 @@ __device__ int wg_isum(int v, Shared &sh)
 // RKF45 attempt (Simulation.cuh:692-740; kernels.cuh:68-199), workgroup-wide.
 // Returns the number of species passing the test; *mind = min of the step
 // factors with thrust::min_element semantics (a NaN first element wins,
-// other NaNs are ignored).
-__device__ int rkf_attempt_wg(const DevModel &m, double rtol, const CtlView &v, double dt, double *mind,
+// other NaNs are ignored).  T is the state type: stage values, k_i, w1, w2,
+// R and the step factors are T; the stage combinations have double literals
+// and are evaluated in double, as in the reference kernels.
+template <typename T>
+__device__ int rkf_attempt_wg(const DevModel &m, double rtol, const CtlViewT<T> &v, T dt, double *mind,
                               Shared &sh)
 {
     const int S = m.S;
     const int tid = threadIdx.x;
-    const double *y1 = v.y1;
-    double *ua = v.ua, *ub = v.ub;
-    double *k1 = v.k1, *k2 = v.k2, *k3 = v.k3, *k4 = v.k4, *k5 = v.k5;
+    const T *y1 = v.y1;
+    T *ua = v.ua, *ub = v.ub;
+    T *k1 = v.k1, *k2 = v.k2, *k3 = v.k3, *k4 = v.k4, *k5 = v.k5;
     const double *fd = m.feed;
 
     for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, v.recs, v.hptr, y1, s);
-        if (fd[s] != 0.0) {
-            k1[s] = fd[s];
-            ua[s] = fd[s];
+        const T f = rhs_species(m, v.recs, v.hptr, y1, s);
+        const T fv = (T)fd[s];
+        if (fv != 0.0) {
+            k1[s] = fv;
+            ua[s] = fv;
         } else {
-            const double kk = f * dt;
+            const T kk = f * dt;
             k1[s] = kk;
             ua[s] = y1[s] + kk / 4.0;
         }
     }
     __syncthreads();
     for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, v.recs, v.hptr, ua, s);
-        if (fd[s] != 0.0) {
-            k2[s] = fd[s];
-            ub[s] = fd[s];
+        const T f = rhs_species(m, v.recs, v.hptr, (const T *)ua, s);
+        const T fv = (T)fd[s];
+        if (fv != 0.0) {
+            k2[s] = fv;
+            ub[s] = fv;
         } else {
-            const double kk = f * dt;
+            const T kk = f * dt;
             k2[s] = kk;
             ub[s] = y1[s] + (3.0 * k1[s]) / 32.0 + (9.0 * kk) / 32.0;
         }
     }
     __syncthreads();
     for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, v.recs, v.hptr, ub, s);
-        if (fd[s] != 0.0) {
-            k3[s] = fd[s];
-            ua[s] = fd[s];
+        const T f = rhs_species(m, v.recs, v.hptr, (const T *)ub, s);
+        const T fv = (T)fd[s];
+        if (fv != 0.0) {
+            k3[s] = fv;
+            ua[s] = fv;
         } else {
-            const double kk = f * dt;
+            const T kk = f * dt;
             k3[s] = kk;
             ua[s] = y1[s] + (1932.0 * k1[s]) / 2197.0 - (7200.0 * k2[s]) / 2197.0 + (7296.0 * kk) / 2197.0;
         }
     }
     __syncthreads();
     for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, v.recs, v.hptr, ua, s);
-        if (fd[s] != 0.0) {
-            k4[s] = fd[s];
-            ub[s] = fd[s];
+        const T f = rhs_species(m, v.recs, v.hptr, (const T *)ua, s);
+        const T fv = (T)fd[s];
+        if (fv != 0.0) {
+            k4[s] = fv;
+            ub[s] = fv;
         } else {
-            const double kk = f * dt;
+            const T kk = f * dt;
             k4[s] = kk;
             ub[s] = y1[s] + (439.0 * k1[s]) / 216.0 - (8.0 * k2[s]) + (3680.0 * k3[s]) / 513.0
                     - (845.0 * kk) / 4104.0;
@@ -164,12 +180,13 @@ __device__ int rkf_attempt_wg(const DevModel &m, double rtol, const CtlView &v, 
     }
     __syncthreads();
     for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, v.recs, v.hptr, ub, s);
-        if (fd[s] != 0.0) {
-            k5[s] = fd[s];
-            ua[s] = fd[s];
+        const T f = rhs_species(m, v.recs, v.hptr, (const T *)ub, s);
+        const T fv = (T)fd[s];
+        if (fv != 0.0) {
+            k5[s] = fv;
+            ua[s] = fv;
         } else {
-            const double kk = f * dt;
+            const T kk = f * dt;
             k5[s] = kk;
             ua[s] = y1[s] - (8.0 * k1[s]) / 27.0 + (2.0 * k2[s]) - (3544.0 * k3[s]) / 2565.0
                     + (1859.0 * k4[s]) / 4104.0 - (11.0 * kk) / 40.0;
@@ -179,24 +196,26 @@ __device__ int rkf_attempt_wg(const DevModel &m, double rtol, const CtlView &v, 
     int nok = 0;
     double dmin = INFINITY;
     for (int s = tid; s < S; s += NT) {
-        const double f = rhs_species(m, v.recs, v.hptr, ua, s);
-        double w1, w2;
-        if (fd[s] != 0.0) {
-            w1 = fd[s];
-            w2 = fd[s];
+        const T f = rhs_species(m, v.recs, v.hptr, (const T *)ua, s);
+        const T fv = (T)fd[s];
+        T w1, w2;
+        if (fv != 0.0) {
+            w1 = fv;
+            w2 = fv;
         } else {
-            const double kk6 = f * dt;
+            const T kk6 = f * dt;
             w1 = y1[s] + (25.0 * k1[s]) / 216.0 + (1408.0 * k3[s]) / 2565.0 + (2197.0 * k4[s]) / 4104.0
                  - k5[s] / 5.0;
             w2 = y1[s] + (16.0 * k1[s]) / 135.0 + (6656.0 * k3[s]) / 12825.0
                  + (28561.0 * k4[s]) / 56430.0 - (9.0 * k5[s] / 50.0) + (2.0 * kk6) / 55.0;
         }
         v.w1[s] = w1;
-        const double R = fabs(w1 - w2) / dt;
-        const double tol = (rtol > 0.0) ? m.atol[s] + rtol * fabs(w1) : m.atol[s];
-        const double d = 0.84 * sqrt(sqrt(tol / R));
+        const T R = fabs(w1 - w2) / dt;
+        const T at = (T)m.atol[s];
+        const T tol = (rtol > 0.0) ? (T)(at + rtol * fabs(w1)) : at;
+        const T d = 0.84 * sqrt_t(sqrt_t(tol / R));
         if (s == 0) sh.iscal[0] = isnan(d) ? 1 : 0;
-        dmin = fmin(dmin, d);
+        dmin = fmin(dmin, (double)d);
         if (R <= tol && !isnan(R) && !isinf(R) && w1 > 0) nok++;
     }
     nok = wg_isum(nok, sh);
@@ -211,17 +230,18 @@ __device__ int rkf_attempt_wg(const DevModel &m, double rtol, const CtlView &v, 
 // (aggiorna_Jacobiano1, kernels.cuh:393-402) for row i, into the dense M
 // (row- or column-major) that was zero-filled beforehand.  Rows are owned by one thread, so
 // the accumulation order per entry is the reference's (reaction ascending).
-template <bool ROW_MAJOR>
-__device__ void jac_row(const DevModel &m, const double *__restrict__ x, double *__restrict__ M, int n,
-                        int i, double h)
+// T is the state and matrix type; each term is accumulated in double and
+// added to the T entry, and m.hk holds sign * k formed in T.
+template <bool ROW_MAJOR, typename T>
+__device__ void jac_row(const DevModel &m, const T *__restrict__ x, T *__restrict__ M, int n, int i, T h)
 {
-    auto at = [&](int row, int col) -> double * {
+    auto at = [&](int row, int col) -> T * {
         return ROW_MAJOR ? M + (size_t)row * n + col : M + (size_t)col * n + row;
     };
     const int e0 = m.hptr[i], e1 = m.hptr[i + 1];
     for (int e = e0; e < e1; e++) {
         const int r = m.hrx[e];
-        const double sk = m.hk[e];
+        const T sk = (T)m.hk[e];
         const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
         for (int j = a0; j < a1; j++) {
             const int g = m.lsp[j];
@@ -230,17 +250,17 @@ __device__ void jac_row(const DevModel &m, const double *__restrict__ x, double 
             if (ex == 1)
                 temp = temp + sk;
             else if (ex > 1)
-                temp = temp + (sk * (double)ex) * mypow(x[g], ex - 1);
+                temp = temp + (sk * (T)ex) * mypow_t(x[g], ex - 1);
             else
                 continue;
             for (int jj = a0; jj < a1; jj++)
                 if (m.lsp[jj] != g) temp = temp * x[m.lsp[jj]];
-            double *p = at(i, g);
-            *p = *p + temp;
+            T *p = at(i, g);
+            *p = (T)(*p + temp);
         }
     }
-    double *d = at(i, i);
-    *d = 1.0 - h * *d;
+    T *d = at(i, i);
+    *d = (T)1.0 - h * *d;
     for (int e = e0; e < e1; e++) {
         const int r = m.hrx[e];
         const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
@@ -260,8 +280,8 @@ __device__ void jac_row(const DevModel &m, const double *__restrict__ x, double 
                 }
             }
             if (dup) continue;
-            double *p = at(i, g);
-            *p = 0.0 - h * *p;
+            T *p = at(i, g);
+            *p = (T)0.0 - h * *p;
         }
     }
 }
@@ -677,5 +697,11 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
 
 
 __device__ __forceinline__ bool bad_value(double v) { return isnan(v) || isinf(v) || v < 0; }
+
+// One term of the Newton norm (Simulation.cuh:58-64): the square returned as
+// float by the reference's functor; summed in double for the FP64 loop and
+// in float (thrust::plus<float>) for the single-precision loop.
+__device__ __forceinline__ double norm_square(double d) { return (double)(float)(d * d); }
+__device__ __forceinline__ float norm_square(float d) { return d * d; }
 
 }  // namespace lassie
