@@ -95,6 +95,23 @@ def test_solution_file_matches_oracle(tmp_path, gpu_lib):
 
 
 @pytest.mark.gpu
+def test_precision_float_runs_the_single_precision_loop(tmp_path, gpu_lib):
+    """-p float: Simulation<float> (simulator.cu:259-304); the file holds the
+    single-precision oracle's trajectory, times as float values."""
+    m = synth.small_nonstiff(0)
+    t = synth.t_vector(0, 10, 100)
+    d = model_folder(tmp_path, m, t=t, cs=[0, 5, 31])
+    print("[cli] 32x48 model, -p float", flush=True)
+    rc, log = run(str(d), str(tmp_path / "out"), "-p", "float")
+    assert rc == 0, log
+    o_out, o_st, o_rc, prog, _ = O.solve32(m, t, cs=np.array([0, 5, 31]), progress_cap=1 << 20)
+    t32 = t.astype(np.float32).astype(np.float64)
+    assert (tmp_path / "out" / "output" / "Solution").read_text() == _fmt(t32, o_out)
+    lines = (tmp_path / "out" / "output" / "ProgressSimulation").read_text().splitlines()
+    assert lines == ["%.4e" % v for v in prog]
+
+
+@pytest.mark.gpu
 def test_stochastic_conversion_matches_oracle(tmp_path, gpu_lib):
     m = cases.mixed()
     vol, avo = 1e-15, 6.022e23
