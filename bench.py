@@ -154,6 +154,7 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="instances in the batched-form run")
     ap.add_argument("--plain-subsample", type=int, default=8)
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form measurement")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the single-precision loop (-p float)")
     ap.add_argument("--diff-batch", type=int, default=256)
     ap.add_argument("--diff-t-end", type=float, default=0.2)
     args = ap.parse_args()
@@ -224,6 +225,22 @@ def main():
             **unit}
         line["max_rel_err_vs_lsoda"] = err_max
         line["median_rel_err_vs_lsoda"] = err_med
+    if not args.no_fp32:
+        say("single-precision loop (-p float) on the same model")
+        dm = _lib.DeviceModel(m)
+        t0 = time.perf_counter()
+        out32, _, st32, rc32, _ = dm.solve(t, precision=_lib.PREC_FP32_LOOP, **CONTROLLER)
+        el32 = time.perf_counter() - t0
+        dm.close()
+        rel = np.abs(out32 - out) / (np.abs(out) + m.atol[None, :])
+        line["fp32_loop"] = {
+            "solves_per_s": 1.0 / el32, "kernel_ms": st32["kernel_ms"],
+            "lu_ms_per_factorization": st32["phase_ms"][2] / max(st32["lu_factorizations"], 1),
+            "bdf_steps": st32["bdf_steps"], "newton_iters": st32["newton_iters"],
+            "max_rel_err_vs_fp64": float(rel.max()), "median_rel_err_vs_fp64": float(np.median(rel)),
+            "note": "Simulation<float> semantics (the reference's -p float), bit-identical to the "
+                    "single-precision oracle"}
+        say(f"single-precision loop done: {el32:.2f} s")
     if not args.no_chosen:
         say(f"batched form, {args.batch} uneven instances; plain path on {args.plain_subsample}")
         line["chosen_form"] = chosen_form(args.batch, args.plain_subsample, True)
