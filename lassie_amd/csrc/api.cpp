@@ -475,13 +475,28 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     // opts.reserved bit 1 keeps the control group's model and vectors in
     // global memory (diagnostics)
     const bool lds_model = model_lds + static_lds <= (size_t)max_lds && !(o.reserved & 2);
-    const size_t dyn = lds_model ? model_lds : sizeof(double) * (size_t)S;
+    size_t dyn = lds_model ? model_lds : sizeof(double) * (size_t)S;
     int per_cu = 0;
     if (lassie::solver_occupancy(S, fp32, dyn, &per_cu) != 0 || per_cu < 1)
         return fail(LASSIE_ERR_HIP, "cannot size the cooperative launch");
     // opts.reserved bits 8..15 cap the number of groups (diagnostics)
     const int gcap = (o.reserved >> 8) & 0xff;
-    const int G = std::max(1, std::min({P, cus * per_cu, gcap ? gcap : 64}));
+    int G = std::max(1, std::min({P, cus * per_cu, gcap ? gcap : 64}));
+    // FP64 with 16-column panels and one strip per group: groups >= 1 keep
+    // their strip in LDS for the whole factorization (the dynamic LDS that
+    // group 0 uses for the model).  opts.reserved bit 2 turns this off
+    // (diagnostics).
+    const int strip_ld = S + 2;
+    bool lds_strip = false;
+    if (!fp32 && nb == 16 && G == P && P > 1 && !(o.reserved & 4)) {
+        const size_t dyn2 = std::max(dyn, sizeof(double) * 16 * (size_t)strip_ld);
+        int pc = 0;
+        if (lassie::solver_occupancy(S, fp32, dyn2, &pc) == 0 && pc >= 1 && std::min(P, cus * pc) == P) {
+            dyn = dyn2;
+            per_cu = pc;
+            lds_strip = true;
+        }
+    }
     lassie::SolveArgs a{};
     a.m.S = S;
     a.m.R = m->R;
@@ -529,6 +544,8 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.lds_model = lds_model ? 1 : 0;
     a.fp32 = fp32;
     a.yfin = m->d_vec + 6 * (size_t)S;  // past the 11 float vectors of the single-precision loop
+    a.lds_strip = lds_strip ? 1 : 0;
+    a.strip_ld = strip_ld;
 
     HIP_TRY(hipEventRecord(m->ev0, m->stream));
     HIP_TRY(lassie::launch_solve(a, dyn, m->stream));

@@ -61,6 +61,8 @@ struct SolveArgs {
     int G;                 // workgroups in the cooperative launch
     int lds_model;         // 1: control group keeps model + vectors in LDS
     int fp32;              // 1: single-precision loop (state, time, M and LU in float)
+    int lds_strip;         // 1: groups >= 1 keep their LU strip in dynamic LDS (FP64, one strip per group)
+    int strip_ld;          // leading dimension of an LDS strip (elements)
     double *yfin;          // fp32 only: final state widened to double (S)
 };
 

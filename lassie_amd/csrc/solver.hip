@@ -257,19 +257,170 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
 // processed without per-row control flow, so a smaller RPT is less work.  The
 // rows each thread holds, and so every result, are the same for any RPT that
 // covers them.
-template <int RPT, int NBC, typename T>
-__device__ void panel_fit(T *M, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
-                          int *info, Shared &sh)
+template <int RPT, int NBC, typename T, typename Store>
+__device__ void panel_fit(const Store st, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
+                          int *rowcnt, int *info, Shared &sh)
 {
     if constexpr (RPT > 1) {
         if (n - p * NBC <= NT * (RPT / 2)) {
-            panel_fit<RPT / 2, NBC, T>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+            panel_fit<RPT / 2, NBC, T>(st, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
             return;
         }
     }
     // row lists feed the vector strip update only
     constexpr bool kList = !(NBC == 16 && sizeof(T) == 8);
-    panel_factor<RPT, NBC, T, false, kList>(M, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+    panel_factor_s<RPT, NBC, T, kList>(st, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+}
+
+// ---------------------------------------------------------------------------
+// LDS-resident strips (FP64, 16-column panels, one strip per workgroup).  The
+// owner of strip q keeps it in LDS for the whole factorization: column-major,
+// leading dimension ld, column c at Ls[c*ld].  Only the factored panel goes to
+// the global matrix at once (the other groups read L11/L21 from there); the
+// rest of the strip is written back when the factorization ends.
+typedef __attribute__((address_space(3))) double LDouble;
+__device__ __forceinline__ int tid_of() { return threadIdx.x; }
+
+// strip_update on an LDS strip: same operations and order as strip_update's
+// MFMA path; the accumulator tiles are read from and written to LDS.  The
+// L21 operands come from the global matrix: every tile's operands are
+// requested on entry (panel p is final), so their latency overlaps the
+// gather and the U12 step.  Panels are never the last one here (q > p), so
+// they have 16 columns; LDS strips are used for n <= 1024, so a wave has at
+// most kMaxTiles row tiles.
+constexpr int kMaxTiles = 16;
+
+__device__ void strip_update_lds(const double *__restrict__ M, LDouble *Ls, int ld, int n, int p, int q,
+                                 const int *perm_src, const int *moved_to, Shared &sh)
+{
+    constexpr int NBC = 16;
+    typedef Mfma<double>::acc Acc;
+    typedef __attribute__((address_space(1))) const double GD;
+    GD *Mg = (GD *)M;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int r0 = p * NBC;
+    const int c0 = q * NBC;
+    const int ncols = min(NBC, n - c0);
+    const int rb = r0 + NBC;
+    const int nt = (n - rb + 15) >> 4;
+    const int ntw = (nt - wave + NWAVES - 1) / NWAVES;  // tiles of this wave, <= kMaxTiles
+    const size_t last = (size_t)n * n - 1;
+    // global loads, in the order they are needed: L11, the pivot-row sources
+    // and displaced-row targets, then the L21 operands
+    const int ek = tid / NBC, ec = tid % NBC;
+    const double l11 = M[(size_t)(r0 + ec) * n + r0 + ek];
+    int src = 0, dst = -1;
+    if (ec < ncols) {
+        src = perm_src[r0 + ek];
+        dst = moved_to[r0 + ek];
+    }
+    // B[k][row] = -L[row][r0 + k] for tile t (rows rb + 16 (wave + 4t) + l16);
+    // padding products are 0 * (-0) = -0, which leaves every value unchanged
+    double b[kMaxTiles][4];
+#pragma unroll
+    for (int t = 0; t < kMaxTiles; t++) {
+        const int r = rb + 16 * (wave + NWAVES * t) + l16;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            const int k = 4 * kc + lq;
+            const bool ok = r < n;
+            const double v = Mg[ok ? (size_t)(r0 + k) * n + r : last];
+            b[t][kc] = ok ? -v : -0.0;
+        }
+    }
+    if (ec < ncols) {
+        sh.U12[ek][ec] = Ls[ec * ld + src];
+        sh.oldP[ek][ec] = Ls[ec * ld + r0 + ek];
+    }
+    sh.L11[ek][ec] = l11;
+    __syncthreads();
+    // displaced panel-range rows go to the positions their pivots vacated
+    if (dst >= rb) Ls[ec * ld + dst] = sh.oldP[ek][ec];
+    // U12 = L11^-1 * A12 (unit lower), one column per thread, k ascending
+    if (tid < ncols) {
+        const int c = tid;
+        double u[NBC];
+#pragma unroll
+        for (int kk = 0; kk < NBC; kk++) u[kk] = sh.U12[kk][c];
+#pragma unroll
+        for (int i = 1; i < NBC; i++) {
+            double v = u[i];
+#pragma unroll
+            for (int kk = 0; kk < i; kk++) v = fma(-sh.L11[i][kk], u[kk], v);
+            u[i] = v;
+        }
+#pragma unroll
+        for (int kk = 0; kk < NBC; kk++) {
+            sh.U12[kk][c] = u[kk];
+            Ls[c * ld + r0 + kk] = u[kk];
+        }
+    }
+    __syncthreads();
+    // trailing update A22 -= L21 U12 over every row below the panel, computed
+    // transposed on the matrix cores (see strip_update), four tiles at a time
+    double ua[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) ua[kc] = (l16 < ncols) ? sh.U12[4 * kc + lq][l16] : 0.0;
+#pragma unroll
+    for (int g = 0; g < kMaxTiles; g += 4) {
+        if (g < ntw) {
+            Acc d[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = rb + 16 * (wave + NWAVES * (g + u)) + l16;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int c = Mfma<double>::row(lq, i);
+                    const bool ok = r < n && c < ncols;
+                    const double v = Ls[ok ? c * ld + r : 0];
+                    d[u][i] = ok ? v : 0.0;
+                }
+            }
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) d[u] = Mfma<double>::run(ua[kc], b[g + u][kc], d[u]);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = rb + 16 * (wave + NWAVES * (g + u)) + l16;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int c = Mfma<double>::row(lq, i);
+                    if (r < n && c < ncols) Ls[c * ld + r] = d[u][i];
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// strip_moves on an LDS strip (row moves of panel p, q < p)
+__device__ void strip_moves_lds(LDouble *Ls, int ld, int n, int p, int q, const int *perm_src,
+                                const int *moved_to, Shared &sh)
+{
+    constexpr int NBC = 16;
+    const int tid = threadIdx.x;
+    const int r0 = p * NBC;
+    const int nbp = min(NBC, n - r0);
+    const int ncols = min(NBC, n - q * NBC);
+    const int kk = tid / NBC, c = tid % NBC;
+    const bool ok = kk < nbp && c < ncols;
+    int src = 0, dst = -1;
+    if (ok) {
+        src = perm_src[r0 + kk];
+        dst = moved_to[r0 + kk];
+        sh.U12[kk][c] = Ls[c * ld + src];
+        sh.oldP[kk][c] = Ls[c * ld + r0 + kk];
+    }
+    __syncthreads();
+    if (ok) {
+        Ls[c * ld + r0 + kk] = sh.U12[kk][c];
+        if (dst >= r0 + nbp) Ls[c * ld + dst] = sh.oldP[kk][c];
+    }
+    __syncthreads();
 }
 
 // Blocked right-looking LU over all groups; strips are owned round-robin.
@@ -293,8 +444,25 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
         const int inf = ld_rlx(&A.ctl->info);
         return inf != 0 && inf - 1 < col_end;
     };
+    // strip wg in LDS (one strip per group; group 0's strip 0 stays in the
+    // global matrix: after the first panel it only receives row moves)
+    constexpr bool kLdsCapable = NBC == 16 && sizeof(T) == 8;
+    const bool ldsq = kLdsCapable && A.lds_strip && wg >= 1 && wg < P;
+    const int ld = A.strip_ld;
+    extern __shared__ __align__(16) unsigned char dyn[];
+    LDouble *Ls = (LDouble *)dyn;
+    if (ldsq) {
+        const int c0 = wg * NBC;
+        const int ncols = min(NBC, n - c0);
+        for (int e = tid_of(); e < ncols * n; e += NT) {
+            const int c = e / n, r = e - c * n;
+            Ls[c * ld + r] = reinterpret_cast<const double *>(M)[(size_t)(c0 + c) * n + r];
+        }
+        __syncthreads();
+    }
     if (wg == 0)
-        panel_fit<RPT, NBC, T>(M, n, 0, A.piv, A.perm_src, A.moved_to, A.rows, A.rowcnt, &A.ctl->info, sh);
+        panel_fit<RPT, NBC, T>(MatStore<T, false>{M, n}, n, 0, A.piv, A.perm_src, A.moved_to, A.rows, A.rowcnt,
+                               &A.ctl->info, sh);
     if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
     if (failed_before(NBC)) {
         singular = true;
@@ -305,15 +473,42 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
         const int *rows = A.rows + (size_t)p * n;
         int q = p + 1 + ((wg - (p + 1)) % G + G) % G;
         for (; q < P; q += G) {
+            if constexpr (kLdsCapable) {
+                if (ldsq) {
+                    const double *Md = reinterpret_cast<const double *>(M);
+                    strip_update_lds(Md, Ls, ld, n, p, q, A.perm_src, A.moved_to, sh);
+                    if (q == p + 1)
+                        panel_fit<RPT, NBC, T>(LdsStripStore<T>{(LDouble *)Ls, ld, q * NBC, M, n}, n, p + 1, A.piv,
+                                               A.perm_src, A.moved_to, A.rows + (size_t)(p + 1) * n,
+                                               A.rowcnt + p + 1, &A.ctl->info, sh);
+                    __syncthreads();
+                    continue;
+                }
+            }
             strip_update<NBC>(M, n, p, q, A.perm_src, A.moved_to, rows, cnt, sh);
             if (q == p + 1) {
                 __syncthreads();
-                panel_fit<RPT, NBC, T>(M, n, p + 1, A.piv, A.perm_src, A.moved_to, A.rows + (size_t)(p + 1) * n,
-                                       A.rowcnt + p + 1, &A.ctl->info, sh);
+                panel_fit<RPT, NBC, T>(MatStore<T, false>{M, n}, n, p + 1, A.piv, A.perm_src, A.moved_to,
+                                       A.rows + (size_t)(p + 1) * n, A.rowcnt + p + 1, &A.ctl->info, sh);
                 __syncthreads();
             }
         }
-        for (int ql = wg; ql < p; ql += G) strip_moves<NBC>(M, n, p, ql, A.perm_src, A.moved_to, sh);
+        for (int ql = wg; ql < p; ql += G) {
+            if (ldsq)
+                strip_moves_lds(Ls, ld, n, p, ql, A.perm_src, A.moved_to, sh);
+            else
+                strip_moves<NBC>(M, n, p, ql, A.perm_src, A.moved_to, sh);
+        }
+        if (ldsq && p == P - 1) {
+            // the strip in its final row order, for the triangular solves
+            double *Mw = reinterpret_cast<double *>(M);
+            const int c0 = wg * NBC;
+            const int ncols = min(NBC, n - c0);
+            for (int e = tid_of(); e < ncols * n; e += NT) {
+                const int c = e / n, r = e - c * n;
+                Mw[(size_t)(c0 + c) * n + r] = Ls[c * ld + r];
+            }
+        }
         if (!grid_barrier(A.gs, G, gen_local, &A.ctl->abort)) return false;
         if (p + 1 < P && failed_before((p + 2) * NBC)) {
             singular = true;

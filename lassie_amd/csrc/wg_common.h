@@ -365,13 +365,58 @@ __device__ __forceinline__ float readlane_d(float v, int lane)
 // LIST only): the only rows the trailing update changes.  T is the element type (double
 // for the FP64 method, float for the precision form); RM selects row-major
 // (else column-major) storage.
+// Where the panel's columns live.  MatStore: the n x n matrix itself (row- or
+// column-major).  LdsStripStore: an LDS copy of the column strip [c0, c0+NBC)
+// (column-major, leading dimension ld); stores also go to the global
+// column-major matrix, where the other workgroups read the factored panel.
+template <typename T, bool RM>
+struct MatStore {
+    T *M;
+    int n;
+    __device__ __forceinline__ T load(int row, int col) const
+    {
+        return RM ? M[(size_t)row * n + col] : M[(size_t)col * n + row];
+    }
+    __device__ __forceinline__ void store(int row, int col, T v) const
+    {
+        if (RM)
+            M[(size_t)row * n + col] = v;
+        else
+            M[(size_t)col * n + row] = v;
+    }
+};
+
+template <typename T>
+struct LdsStripStore {
+    typedef __attribute__((address_space(3))) T LT;
+    LT *L;
+    int ld, c0;
+    T *M;
+    int n;
+    __device__ __forceinline__ T load(int row, int col) const { return L[(col - c0) * ld + row]; }
+    __device__ __forceinline__ void store(int row, int col, T v) const
+    {
+        L[(col - c0) * ld + row] = v;
+        M[(size_t)col * n + row] = v;
+    }
+};
+
+template <int RPT, int NBC, typename T, bool LIST, typename Store>
+__device__ void panel_factor_s(const Store st, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
+                               int *rowcnt, int *info, Shared &sh);
+
 template <int RPT, int NBC, typename T, bool RM, bool LIST = true>
 __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *perm_src, int *moved_to,
                              int *rows, int *rowcnt, int *info, Shared &sh)
 {
-    auto at = [&](int row, int col) -> T & {
-        return RM ? M[(size_t)row * n + col] : M[(size_t)col * n + row];
-    };
+    panel_factor_s<RPT, NBC, T, LIST>(MatStore<T, RM>{M, n}, n, p, piv, perm_src, moved_to, rows, rowcnt, info,
+                                      sh);
+}
+
+template <int RPT, int NBC, typename T, bool LIST, typename Store>
+__device__ void panel_factor_s(const Store st, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
+                               int *rowcnt, int *info, Shared &sh)
+{
     const int r0 = p * NBC;
     const int nbp = min(NBC, n - r0);
     const int tid = threadIdx.x;
@@ -386,7 +431,7 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
         pos[q] = r;
         act[q] = r < n;
 #pragma unroll
-        for (int c = 0; c < NBC; c++) a[q][c] = (r < n && c < nbp) ? at(r, r0 + c) : T(0);
+        for (int c = 0; c < NBC; c++) a[q][c] = (r < n && c < nbp) ? st.load(r, r0 + c) : T(0);
     }
     int buf = 0;
     bool stop = false;
@@ -498,7 +543,7 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
             if (orig < n) {
 #pragma unroll
                 for (int c = 0; c < NBC; c++)
-                    if (c < nbp) at(pos[q], r0 + c) = a[q][c];
+                    if (c < nbp) st.store(pos[q], r0 + c, a[q][c]);
                 if (!act[q]) perm_src[pos[q]] = orig;
                 if (orig < r0 + nbp) moved_to[orig] = pos[q];
             }
@@ -537,7 +582,7 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
         if (orig < n) {
 #pragma unroll
             for (int c = 0; c < NBC; c++)
-                if (c < nbp) at(pos[q], r0 + c) = a[q][c];
+                if (c < nbp) st.store(pos[q], r0 + c, a[q][c]);
             if (!act[q]) perm_src[pos[q]] = orig;
             if (orig < r0 + nbp) moved_to[orig] = pos[q];
             if (nzq[q]) rows[base[q] + off] = pos[q];

@@ -112,6 +112,29 @@ def test_precision_float_runs_the_single_precision_loop(tmp_path, gpu_lib):
 
 
 @pytest.mark.gpu
+def test_sbml_import_then_lassie(tmp_path, gpu_lib):
+    """SBML -> BioSimWare (lassie_amd/sbml.py) -> LASSIE: the Solution file
+    equals the oracle's trajectory of the converted model (its constant
+    species is a feed).  Kept with the other command-line runs: a child
+    process cannot make a cooperative launch while this process holds the
+    GPU's cooperative queue (after its own solves)."""
+    from lassie_amd import sbml
+    from tests.test_sbml import L2_DOC
+    m = sbml.parse(L2_DOC)
+    t = np.linspace(0.0, 2.0, 11)
+    sbml.write_biosimware(m, str(tmp_path / "in"), t=t)
+    print("[cli] SBML toy model", flush=True)
+    rc, log = run(str(tmp_path / "in"), str(tmp_path / "out"))
+    assert rc == 0, log
+    model = sbml.to_model(m)
+    model.k = np.loadtxt(tmp_path / "in" / "c_vector")          # as the reader sees them ("%e")
+    model.x0 = np.loadtxt(tmp_path / "in" / "M_0", delimiter="\t")
+    out, st, o_rc, _, _ = O.solve(model, t, max_iters=2_000_000)
+    assert o_rc == 0
+    assert (tmp_path / "out" / "output" / "Solution").read_text() == _fmt(t, out)
+
+
+@pytest.mark.gpu
 def test_stochastic_conversion_matches_oracle(tmp_path, gpu_lib):
     m = cases.mixed()
     vol, avo = 1e-15, 6.022e23

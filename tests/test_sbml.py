@@ -159,24 +159,3 @@ def test_command_line(tmp_path):
     assert (tmp_path / "out" / "t_vector").read_text().split() == ["0", "2.5", "5", "7.5", "10"]
     assert sbml.main([str(tmp_path / "missing.sbml"), str(tmp_path / "o2")]) == 2
 
-
-@pytest.mark.gpu
-def test_converted_model_runs_through_the_lassie_command_line(tmp_path, gpu_lib):
-    """SBML -> BioSimWare -> LASSIE: the Solution file equals the oracle's
-    trajectory of the converted model (the constant species is a feed)."""
-    import subprocess
-    m = sbml.parse(L2_DOC)
-    t = np.linspace(0.0, 2.0, 11)
-    sbml.write_biosimware(m, str(tmp_path / "in"), t=t)
-    exe = os.path.join(os.path.dirname(HERE), "lassie_amd", "LASSIE")
-    print("[sbml] toy model through the LASSIE command line", flush=True)
-    p = subprocess.run([exe, str(tmp_path / "in"), str(tmp_path / "out")], capture_output=True, text=True,
-                       timeout=120)
-    assert p.returncode == 0, p.stdout + p.stderr
-    model = sbml.to_model(m)
-    model.k = np.loadtxt(tmp_path / "in" / "c_vector")          # as the reader sees them ("%e")
-    model.x0 = np.loadtxt(tmp_path / "in" / "M_0", delimiter="\t")
-    out, st, rc, _, _ = O.solve(model, t, max_iters=2_000_000)
-    assert rc == 0
-    expect = "".join("%.4e\t" % ti + "".join("%.4e\t" % v for v in row) + "\n" for ti, row in zip(t, out))
-    assert (tmp_path / "out" / "output" / "Solution").read_text() == expect
