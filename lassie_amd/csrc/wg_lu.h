@@ -21,6 +21,12 @@ __device__ __forceinline__ float to_bf16_rne(float x)
 // writable elements.  RT: row tiles of the trailing update in flight per wave.  Row moves go to every column,
 // so on return M holds L\U in final row order.  Returns 0 or k+1 of the
 // first zero pivot.  BF16 = round the trailing-update operands to BF16.
+constexpr int kLpStride = 17;  // LDS row stride of the staged L21 panel (elements)
+
+// L21 of the current panel (at most kBatchMaxS - kBatchNB rows), shared by
+// every precision's instantiation of wg_lu_rm in a kernel
+static __shared__ double lp_raw[(kBatchMaxS - kBatchNB) * kLpStride];
+
 template <typename T, bool BF16, int RT = 4>
 __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
                         int *info, Shared &sh)
@@ -81,9 +87,22 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
         // accumulates as a sequential fma chain, the oracle's order).  Every
         // row is updated, as in the oracle.  A wave owns the column tiles
         // ct = wave (mod 4), keeps their U12 operands in registers and walks
-        // the row tiles RT at a time, the next RT tiles' operands in flight.
+        // the row tiles RT at a time, the next RT tiles' output tiles in
+        // flight.  The L21 operands, which every column tile reuses, are
+        // staged once in LDS (negated, and rounded for BF16), kLpStride
+        // elements per row so that the 16 rows of a tile fall in different
+        // banks.
         const int c0 = r0 + nbp;
         if (c0 < n) {
+            typedef __attribute__((address_space(3))) T LT;
+            LT *Lp = (LT *)lp_raw;
+            for (int e = tid; e < (n - c0) * NBC; e += NT) {
+                const int rr = e / NBC, k = e % NBC;
+                T v = (k < nbp) ? -M[(size_t)(c0 + rr) * n + r0 + k] : T(-0.0);
+                if (BF16) v = (T)to_bf16_rne((float)v);
+                Lp[rr * kLpStride + k] = v;
+            }
+            __syncthreads();
             typedef typename Mfma<T>::acc Acc;
             // global-memory view of M (the workspace lives in device memory):
             // global loads/stores keep the memory counters precise, so the
@@ -125,11 +144,8 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
                         for (int kc = 0; kc < 4; kc++) {
                             const int k = 4 * kc + lq;
                             const bool ok = okt && ra < n && k < nbp;
-                            const T v = Mg[ok ? (size_t)ra * n + r0 + k : last];
                             // padding products are (-0) * 0 = -0, which leaves every value unchanged
-                            T av = ok ? -v : T(-0.0);
-                            if (BF16) av = (T)to_bf16_rne((float)av);
-                            aa[u][kc] = av;
+                            aa[u][kc] = ok ? (T)Lp[(ra - c0) * kLpStride + k] : T(-0.0);
                         }
                     }
                 };
