@@ -83,14 +83,22 @@ def chosen_form(n_batch, n_plain, with_plain):
     """Batched FP64 form (a plan: graph replay, autotuned kernel variant) vs
     the plain path on the same instances."""
     from lassie_amd import _lib, synth
+    t0 = time.perf_counter()
     models = synth.uneven_batch(n_batch, 1000)
     t = synth.t_vector(0.0, 50.0, 100)
-    dms = [_lib.DeviceModel(m) for m in models]
+    say(f"  {n_batch} models generated ({time.perf_counter() - t0:.1f} s)")
+    dms = []
+    for i, m in enumerate(models):
+        dms.append(_lib.DeviceModel(m))
+        if (i + 1) % 256 == 0:
+            say(f"  {i + 1} models uploaded ({time.perf_counter() - t0:.1f} s)")
     plan = _lib.BatchPlan(dms, t, **CONTROLLER)
     variant, tile_rows, trial_ms = plan.info()
+    say(f"  plan prepared, variant {variant} ({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
     outs, _, st, ms, rc = plan.run()
     wall = time.perf_counter() - t0
+    say(f"  batched run: {wall:.2f} s")
     plan.close()
     res = {"instances": n_batch, "sizes": sorted({m.S for m in models}),
            "batched_solves_per_s": n_batch / wall, "batched_device_ms": ms,
