@@ -116,7 +116,13 @@ int lassie_model_destroy(lassie_model *model);
  * receives t/t_max at the top of every loop iteration (ProgressSimulation,
  * Simulation.cuh:660-672).  opts.precision LASSIE_PREC_FP64 runs Simulation<double>,
  * LASSIE_PREC_FP32_LOOP runs Simulation<float> (values widened to double on
- * output).  Reference: Simulation<T>::run, Simulation.cuh:230-981. */
+ * output), LASSIE_PREC_FP32_REFINE runs the FP64 loop with the Newton matrix
+ * factored in FP32 and every Newton solve refined in FP64 (opts.refine_tol,
+ * opts.refine_max_iter; a zero FP32 pivot or a refinement that does not
+ * converge switches that BDF step to the FP64 LU, counted in
+ * stats->refine_fallbacks).  LASSIE_PREC_BF16_REFINE is batched-form only.
+ * Reference: Simulation<T>::run, Simulation.cuh:230-981; the precision form
+ * extends linearSolverLU, Simulation.cuh:1280-1326. */
 int lassie_solve(lassie_model *model, const lassie_opts *opts, const double *t, int32_t nt,
                  const int32_t *cs, int32_t ncs, double *out, double *y_final,
                  double *progress, int64_t progress_cap, lassie_stats *stats);

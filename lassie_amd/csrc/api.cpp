@@ -63,6 +63,10 @@ struct lassie_model {
     double *d_hk32 = nullptr;
     lassie::HRec *d_recs32 = nullptr;
     unsigned long long version32 = ~0ULL;
+    // precision form: FP32 copy of the Newton matrix and the refinement
+    // vectors, allocated on first use
+    float *d_M32 = nullptr;
+    double *d_rvec = nullptr;
     double *d_vec = nullptr;  // work vectors, 12 x S
     double *d_M = nullptr;    // dense S x S Newton matrix (column-major)
     int *d_piv = nullptr, *d_perm = nullptr, *d_moved = nullptr, *d_rows = nullptr, *d_rowcnt = nullptr;
@@ -87,7 +91,7 @@ static void free_model(lassie_model *m)
                     m->d_atol, m->d_vec, m->d_M, m->d_piv, m->d_perm, m->d_moved, m->d_rows, m->d_rowcnt, m->d_ctl, m->d_gs,
                     m->d_stats, m->d_dstats, m->d_tv, m->d_out, m->d_prog, m->d_cs, m->d_recs,
                     m->d_htptr, m->d_htsp, m->d_htcoef, m->d_ltptr, m->d_ltent, m->d_lrx, m->d_k,
-                    m->d_pfull, m->d_tri, m->d_tflag, m->d_hk32, m->d_recs32};
+                    m->d_pfull, m->d_tri, m->d_tflag, m->d_hk32, m->d_recs32, m->d_M32, m->d_rvec};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -430,9 +434,13 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
         o = *opts;
     else
         lassie_opts_default(&o);
-    if (o.precision != LASSIE_PREC_FP64 && o.precision != LASSIE_PREC_FP32_LOOP)
-        return fail(LASSIE_ERR_UNSUPPORTED, "single-model solve supports LASSIE_PREC_FP64 and LASSIE_PREC_FP32_LOOP");
+    if (o.precision != LASSIE_PREC_FP64 && o.precision != LASSIE_PREC_FP32_LOOP &&
+        o.precision != LASSIE_PREC_FP32_REFINE)
+        return fail(LASSIE_ERR_UNSUPPORTED,
+                    "single-model solve supports LASSIE_PREC_FP64, LASSIE_PREC_FP32_REFINE and LASSIE_PREC_FP32_LOOP");
     const int fp32 = o.precision == LASSIE_PREC_FP32_LOOP ? 1 : 0;
+    const int refine = o.precision == LASSIE_PREC_FP32_REFINE ? 1 : 0;
+    if (refine && !(o.refine_tol >= 0.0)) return fail(LASSIE_ERR_INVALID, "refine_tol must be >= 0");
     if (m->S > lassie::kMaxDenseN)
         return fail(LASSIE_ERR_UNSUPPORTED, "dense Newton matrix limited to 4096 species");
     if (progress_cap < 0 || (progress_cap > 0 && !progress)) return fail(LASSIE_ERR_INVALID, "bad progress buffer");
@@ -452,6 +460,17 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
         HIP_TRY(hipMemcpy(m->d_hk32, hk32.data(), sizeof(double) * nh, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(m->d_recs32, recs32.data(), sizeof(lassie::HRec) * nh, hipMemcpyHostToDevice));
         m->version32 = m->version;
+    }
+    if (refine && !m->d_M32) {
+        const size_t S2 = (size_t)m->S * m->S;
+        if (hipMalloc((void **)&m->d_M32, sizeof(float) * (S2 + lassie::kLuSink)) != hipSuccess) {
+            m->d_M32 = nullptr;
+            return fail(LASSIE_ERR_NOMEM, "FP32 Newton matrix");
+        }
+        if (hipMalloc((void **)&m->d_rvec, sizeof(double) * 4 * (size_t)m->S) != hipSuccess) {
+            m->d_rvec = nullptr;
+            return fail(LASSIE_ERR_NOMEM, "refinement vectors");
+        }
     }
     if ((rc = ensure(&m->d_tv, &m->cap_tv, (size_t)nt))) return rc;
     if ((rc = ensure(&m->d_cs, &m->cap_cs, (size_t)ncs))) return rc;
@@ -477,7 +496,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     const bool lds_model = model_lds + static_lds <= (size_t)max_lds && !(o.reserved & 2);
     size_t dyn = lds_model ? model_lds : sizeof(double) * (size_t)S;
     int per_cu = 0;
-    if (lassie::solver_occupancy(S, fp32, dyn, &per_cu) != 0 || per_cu < 1)
+    if (lassie::solver_occupancy(S, fp32, refine, dyn, &per_cu) != 0 || per_cu < 1)
         return fail(LASSIE_ERR_HIP, "cannot size the cooperative launch");
     // opts.reserved bits 8..15 cap the number of groups (diagnostics)
     const int gcap = (o.reserved >> 8) & 0xff;
@@ -491,7 +510,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     if (!fp32 && nb == 16 && G == P && P > 1 && !(o.reserved & 4)) {
         const size_t dyn2 = std::max(dyn, sizeof(double) * 16 * (size_t)strip_ld);
         int pc = 0;
-        if (lassie::solver_occupancy(S, fp32, dyn2, &pc) == 0 && pc >= 1 && std::min(P, cus * pc) == P) {
+        if (lassie::solver_occupancy(S, fp32, refine, dyn2, &pc) == 0 && pc >= 1 && std::min(P, cus * pc) == P) {
             dyn = dyn2;
             per_cu = pc;
             lds_strip = true;
@@ -546,6 +565,11 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.yfin = m->d_vec + 6 * (size_t)S;  // past the 11 float vectors of the single-precision loop
     a.lds_strip = lds_strip ? 1 : 0;
     a.strip_ld = strip_ld;
+    a.refine = refine;
+    a.refine_max_iter = std::max(1, o.refine_max_iter);
+    a.refine_tol = o.refine_tol;
+    a.M32 = m->d_M32;
+    a.rvec = m->d_rvec;
 
     HIP_TRY(hipEventRecord(m->ev0, m->stream));
     HIP_TRY(lassie::launch_solve(a, dyn, m->stream));
@@ -582,6 +606,8 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
         stats->nan_repairs = st[6];
         stats->samples = st[7];
         stats->lu_factorizations = st[8];
+        stats->refine_iters = st[10];
+        stats->refine_fallbacks = st[11];
         stats->t_final = dst[0];
         stats->dt_final = dst[1];
         stats->newdt_final = dst[2];

@@ -26,6 +26,8 @@ struct Ctl {
     unsigned int ngo;    // Newton iteration stamp published by group 0
     int ncont;           // 1: solve this iteration, 0: the Newton loop ends
     int pad5[14];
+    unsigned int rbad;   // precision form: number of the last refinement sweep that missed its target
+    int pad6[15];
 };
 
 constexpr int kTriBlock = 64;     // rows per block of the distributed triangular solves
@@ -56,7 +58,7 @@ struct SolveArgs {
     unsigned int *tflag;   // 2 x blocks x kFlagStride: forward / backward block stamps
     Ctl *ctl;
     GridSync *gs;
-    long long *stats_out;  // 10 counters
+    long long *stats_out;  // 12 counters
     double *dstats_out;    // t, dt, newdt, phase ms x4
     int G;                 // workgroups in the cooperative launch
     int lds_model;         // 1: control group keeps model + vectors in LDS
@@ -64,10 +66,17 @@ struct SolveArgs {
     int lds_strip;         // 1: groups >= 1 keep their LU strip in dynamic LDS (FP64, one strip per group)
     int strip_ld;          // leading dimension of an LDS strip (elements)
     double *yfin;          // fp32 only: final state widened to double (S)
+    // precision form (FP64 loop): the Newton matrix is factored in FP32 and
+    // each Newton solve is refined in FP64 against the unfactored M
+    int refine;            // 1: FP32 LU + FP64 iterative refinement
+    int refine_max_iter;   // sweeps before the FP64 LU of the step's matrix
+    double refine_tol;     // stop when |r_i| <= tol * (|b_i| + (|M||x|)_i) for every i
+    float *M32;            // S x S, column-major like M, followed by kLuSink elements
+    double *rvec;          // 4 x S: right-hand side, residual, iterate (two buffers)
 };
 
 hipError_t launch_solve(const SolveArgs &args, size_t dyn_lds, hipStream_t stream);
-int solver_occupancy(int n, int fp32, size_t dyn_lds, int *per_cu);
+int solver_occupancy(int n, int fp32, int refine, size_t dyn_lds, int *per_cu);
 int dense_panel_width(int n);
 size_t lds_model_bytes(int S, int nnz_h);
 

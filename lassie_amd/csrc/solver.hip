@@ -553,8 +553,10 @@ __device__ __forceinline__ void publish_stamp(unsigned int *flag, unsigned int v
 }
 
 // L y = b for block j (wave 0 of its owner): b = tri[0..S), y -> tri[S..2S).
+// With rsrc, b is read from there instead, in the factors' row order and
+// rounded to T: b_i = (T)rsrc[perm_full[i]].
 template <typename T>
-__device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp)
+__device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp, const double *rsrc = nullptr)
 {
     constexpr int TB = kTriBlock;
     const int n = A.m.S;
@@ -571,7 +573,8 @@ __device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp)
     T d[TB];
 #pragma unroll
     for (int k = 0; k < TB; k++) d[k] = (k < lane && row) ? M[(size_t)(R0 + k) * n + i] : (T)0.0;
-    T acc = row ? bp[i] : (T)0.0;
+    T acc = (T)0.0;
+    if (row) acc = rsrc ? (T)rsrc[A.perm_full[i]] : bp[i];
     for (int B = 0; B < j; B++) {
         const int c0 = B * TB;
         T l[16], ln[16];
@@ -672,9 +675,132 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
 }
 
 // ---------------------------------------------------------------------------
-template <int RPT, int NBC, typename T>
+// Precision form, helpers (FP64 loop; REF instantiation of solve_kernel).
+
+// Final row order of the factors (row i of L U is row perm_full[i] of M): the
+// panels' row moves composed.  One group.
+template <int NBC>
+__device__ void compose_perm(const SolveArgs &A, int n)
+{
+    const int tid = threadIdx.x;
+    int *pf = A.perm_full;
+    for (int s = tid; s < n; s += NT) pf[s] = s;
+    __syncthreads();
+    const int P = (n + NBC - 1) / NBC;
+    for (int p = 0; p < P; p++) {
+        const int r0 = p * NBC;
+        const int nbp = min(NBC, n - r0);
+        int v1 = 0, v2 = 0, dst = -1;
+        if (tid < nbp) {
+            v1 = pf[A.perm_src[r0 + tid]];
+            v2 = pf[r0 + tid];
+            dst = A.moved_to[r0 + tid];
+        }
+        __syncthreads();
+        if (tid < nbp) {
+            pf[r0 + tid] = v1;
+            if (dst >= r0 + nbp) pf[dst] = v2;
+        }
+        __syncthreads();
+    }
+}
+
+// One refinement sweep after the FP32 solve d = tri32[2S..3S): x_new = x_old
+// + (double)d (x_old = 0 on the first sweep), r = rhs - M x_new and the
+// Oettli-Prager test |r_i| <= tol * (|rhs_i| + (|M||x_new|)_i + atol_i), both
+// sums as fma chains over j ascending from 0 (M is the unfactored FP64
+// matrix, column-major); the species' absolute tolerance is the floor that
+// keeps components far below it (under FP32's normal range) from holding
+// refinement back.  Rows go in 64-row chunks to the waves of all groups, a lane
+// per row; x_new arrives 64 entries at a time, broadcast from the lane that
+// formed it.  A group with a row off target raises ctl->rbad to seq.
+__device__ void refine_residual(const SolveArgs &A, int wg, const double *rhs, const double *xo, double *xn,
+                                double *res, unsigned int seq)
+{
+    const int n = A.m.S;
+    const int G = A.G;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const double *Md = A.M;
+    const float *d = reinterpret_cast<const float *>(A.tri) + 2 * (size_t)n;
+    const double tol = A.refine_tol;
+    const int nch = (n + 63) / 64;
+    int bad = 0;
+    for (int c = wg + G * wave; c < nch; c += G * NWAVES) {
+        const int i = c * 64 + lane;
+        const bool row = i < n;
+        double pa = 0.0, pm = 0.0;
+        for (int j0 = 0; j0 < n; j0 += 64) {
+            const int j = j0 + lane;
+            double xj = 0.0;
+            if (j < n) xj = (xo ? xo[j] : 0.0) + (double)d[j];
+            const int jn = min(64, n - j0);
+#pragma unroll
+            for (int sb = 0; sb < 4; sb++) {
+                double mv[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                    const int k = sb * 16 + kk;
+                    mv[kk] = (row && k < jn) ? Md[(size_t)(j0 + k) * n + i] : 0.0;
+                }
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                    const int k = sb * 16 + kk;
+                    if (k < jn) {
+                        const double xk = readlane_d(xj, k);
+                        pa = fma(mv[kk], xk, pa);
+                        pm = fma(fabs(mv[kk]), fabs(xk), pm);
+                    }
+                }
+            }
+        }
+        if (row) {
+            xn[i] = (xo ? xo[i] : 0.0) + (double)d[i];
+            const double b = rhs[i];
+            const double r = b - pa;
+            res[i] = r;
+            const double mag = (fabs(b) + pm) + A.m.atol[i];
+            if (!(fabs(r) <= tol * mag)) bad = 1;
+        }
+    }
+    if (__syncthreads_or(bad) && tid == 0)
+        __hip_atomic_fetch_max(&A.ctl->rbad, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Distributed triangular solves of one right-hand side, b_i = (T)rsrc[perm_full[i]]
+// (T = float: the FP32 factors in A.M; double: the FP64 ones).  Wave 0 of the
+// owners of the row blocks; false on abort.
+template <typename T>
+__device__ bool tri_solve_all(const SolveArgs &A, int wg, unsigned int stamp, const double *rsrc)
+{
+    const int n = A.m.S;
+    const int G = A.G;
+    const int nblk = (n + kTriBlock - 1) / kTriBlock;
+    __shared__ int s_ok;
+    if (threadIdx.x < 64) {
+        bool ok = true;
+        for (int j = wg; j < nblk && ok; j += G) ok = tri_forward_block<T>(A, j, stamp, rsrc);
+        int jl = wg;
+        while (jl + G < nblk) jl += G;
+        if (wg < nblk)
+            for (int j = jl; j >= 0 && ok; j -= G) ok = tri_backward_block<T>(A, j, nblk, stamp);
+        if (threadIdx.x == 0) s_ok = ok ? 1 : 0;
+    }
+    __syncthreads();
+    const bool ok = s_ok != 0;
+    __syncthreads();
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
+// REF: precision form (T = double only): M is rounded to FP32 and factored on
+// the grid; each Newton solve is refined in FP64 (refine_residual); a zero
+// FP32 pivot or a refinement that misses its target within refine_max_iter
+// sweeps switches the step to the FP64 LU of M.
+template <int RPT, int NBC, typename T, bool REF = false>
 __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
 {
+    static_assert(!REF || sizeof(T) == 8, "the precision form runs the FP64 loop");
     extern __shared__ __align__(16) unsigned char dyn[];
     __shared__ Shared sh;
     const int tid = threadIdx.x;
@@ -735,7 +861,8 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
     int i_sample = 0;
     long long nrej = 0;
     long long st_loop = 0, st_att = 0, st_acc = 0, st_bdf = 0, st_newton = 0, st_lufail = 0, st_rep = 0,
-              st_lu = 0;
+              st_lu = 0, st_ref = 0, st_fb = 0;
+    unsigned int rseq = 0;  // refinement sweeps seen (identical in every group)
     int rc = 0;
     // device-time attribution (group 0, 100 MHz wall clock): control/RKF,
     // Jacobian assembly, LU, Newton solves
@@ -833,6 +960,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         const T h = (T)ld_rlx_d(&ctl->h[parity]);
 
         bool singular = false;
+        bool fp64 = true;  // REF: this step solves with the FP64 factors of M
         if (mode == MODE_BDF) {
             // ---- dense M = I - hJ(y_v), all groups -------------------------------
             {
@@ -864,7 +992,29 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 ph[1] += now - tmark;
                 tmark = now;
             }
-            if (!lu_factor_grid<RPT, NBC, T>(A, wg, sh, gen_local, singular)) return;
+            if constexpr (REF) {
+                // M32 = (float)M, then the FP32 LU on the grid
+                const size_t nn = (size_t)n * n;
+                const double *Md = A.M;
+                for (size_t e = (size_t)wg * NT + tid; e < nn; e += (size_t)G * NT) A.M32[e] = (float)Md[e];
+                if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
+                SolveArgs A32 = A;
+                A32.M = reinterpret_cast<double *>(A.M32);
+                A32.lds_strip = 0;
+                if (!lu_factor_grid<RPT, NBC, float>(A32, wg, sh, gen_local, singular)) return;
+                fp64 = singular;
+                if (singular) {
+                    // zero FP32 pivot: the FP64 LU of M for this step.  Every
+                    // group has read ctl->info before it is cleared.
+                    if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
+                    if (wg == 0 && tid == 0) st_rlx(&ctl->info, 0);
+                    if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
+                    st_fb++;
+                    if (!lu_factor_grid<RPT, NBC, T>(A, wg, sh, gen_local, singular)) return;
+                }
+            } else {
+                if (!lu_factor_grid<RPT, NBC, T>(A, wg, sh, gen_local, singular)) return;
+            }
             if (wg == 0) {
                 const unsigned long long now = wall_clock64();
                 ph[2] += now - tmark;
@@ -875,7 +1025,112 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         // Group 0 forms each residual and updates the iterate; the triangular
         // solves are spread over the owners of the row blocks (tri_*_block).
         bool luFail = false;
-        if (mode == MODE_BDF) {
+        if constexpr (REF) {
+            if (mode == MODE_BDF) {
+                // Every group takes part: group 0 forms the residual and updates
+                // the iterate; the FP32 solves, the refinement sweeps and any
+                // FP64 solves are spread over all groups.
+                double *rhs64 = A.rvec;
+                double *rres = A.rvec + Sz;
+                double *xbuf[2] = {A.rvec + 2 * Sz, A.rvec + 3 * Sz};
+                SolveArgs A32 = A;
+                A32.M = reinterpret_cast<double *>(A.M32);
+                if (wg == 0) {
+                    st_lu++;
+                    if (!singular) compose_perm<NBC>(A, n);
+                }
+                int itN = 0;
+                T errN = 1000;
+                while (true) {
+                    if (wg == 0) {
+                        int cont = (itN < A.newton_iter && errN > (T)A.newton_tol) ? 1 : 0;
+                        if (cont && itN > 0) {
+                            for (int s = tid; s < S; s += NT) {
+                                const T f = rhs_species(A.m, v.recs, v.hptr, (const T *)v.yv, s);
+                                v.x[s] = -(v.yv[s] - v.y1[s] - h * f);
+                            }
+                            __syncthreads();
+                        }
+                        if (cont && singular) {
+                            for (int s = tid; s < S; s += NT)
+                                if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
+                            __syncthreads();
+                            newdt = h * 0.1;
+                            luFail = true;
+                            st_lufail++;
+                            cont = 0;
+                        }
+                        if (cont)
+                            for (int s = tid; s < S; s += NT) rhs64[s] = v.x[s];
+                        if (tid == 0) st_rlx(&ctl->ncont, cont);
+                    }
+                    if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
+                    if (!ld_rlx(&ctl->ncont)) break;
+                    const double *xsol = nullptr;
+                    if (!fp64) {
+                        for (int it = 0; it < A.refine_max_iter; it++) {
+                            tri_stamp++;
+                            rseq++;
+                            st_ref++;
+                            if (!tri_solve_all<float>(A32, wg, tri_stamp, it == 0 ? rhs64 : rres)) return;
+                            if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
+                            double *xn = xbuf[(it + 1) & 1];
+                            refine_residual(A, wg, rhs64, it == 0 ? nullptr : xbuf[it & 1], xn, rres, rseq);
+                            if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
+                            if (__hip_atomic_load(&ctl->rbad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != rseq) {
+                                xsol = xn;
+                                break;
+                            }
+                        }
+                        if (!xsol) {
+                            // no convergence: the FP64 LU of M for the rest of the step
+                            fp64 = true;
+                            st_fb++;
+                            if (!lu_factor_grid<RPT, NBC, T>(A, wg, sh, gen_local, singular)) return;
+                            if (singular) {
+                                if (wg == 0) {
+                                    for (int s = tid; s < S; s += NT)
+                                        if (bad_value(v.yv[s])) v.yv[s] = v.y1[s];
+                                    __syncthreads();
+                                    newdt = h * 0.1;
+                                    luFail = true;
+                                    st_lufail++;
+                                }
+                                break;
+                            }
+                            if (wg == 0) compose_perm<NBC>(A, n);
+                            if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
+                        }
+                    }
+                    if (fp64) {
+                        // the last block of the backward chain belongs to group 0,
+                        // which has seen every other block's stamp when it ends
+                        tri_stamp++;
+                        if (!tri_solve_all<T>(A, wg, tri_stamp, rhs64)) return;
+                        xsol = reinterpret_cast<const double *>(A.tri) + 2 * Sz;
+                    }
+                    if (wg == 0) {
+                        __threadfence();
+                        T part = 0.0;
+                        for (int s = tid; s < S; s += NT) {
+                            const T yv1 = v.yv[s];
+                            const T fv = (T)A.m.feed[s];
+                            T yv;
+                            if (fv == 0)
+                                yv = yv1 + xsol[s];
+                            else
+                                yv = fv;
+                            v.yv[s] = yv;
+                            const T d = yv - yv1;
+                            part = part + norm_square(d);
+                        }
+                        errN = sqrt_t(wg_tree_sum(part, sh));
+                        itN = itN + 1;
+                    }
+                }
+                if (wg == 0) st_newton += itN;
+            }
+        } else if (mode == MODE_BDF) {
             const int nblk = (n + kTriBlock - 1) / kTriBlock;
             if (wg == 0) {
                 st_lu++;
@@ -1019,6 +1274,8 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             o[7] = i_sample;
             o[8] = st_lu;
             o[9] = rc;
+            o[10] = st_ref;
+            o[11] = st_fb;
             double *d = A.dstats_out;
             d[0] = t;
             d[1] = dt;
@@ -1036,26 +1293,27 @@ size_t lds_model_bytes(int S, int nnz_h)
 }
 
 template <int RPT, int NBC>
-static const void *kernel_ptr(int fp32)
+static const void *kernel_ptr(int fp32, int refine)
 {
-    return fp32 ? reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, float>)
-                : reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double>);
+    if (fp32) return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, float>);
+    if (refine) return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double, true>);
+    return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double>);
 }
 
-static const void *pick_kernel(int n, int fp32)
+static const void *pick_kernel(int n, int fp32, int refine)
 {
-    if (n <= 256) return kernel_ptr<1, 16>(fp32);
-    if (n <= 512) return kernel_ptr<2, 16>(fp32);
-    if (n <= 1024) return kernel_ptr<4, 16>(fp32);
-    if (n <= 2048) return kernel_ptr<8, 16>(fp32);
-    return kernel_ptr<16, 8>(fp32);
+    if (n <= 256) return kernel_ptr<1, 16>(fp32, refine);
+    if (n <= 512) return kernel_ptr<2, 16>(fp32, refine);
+    if (n <= 1024) return kernel_ptr<4, 16>(fp32, refine);
+    if (n <= 2048) return kernel_ptr<8, 16>(fp32, refine);
+    return kernel_ptr<16, 8>(fp32, refine);
 }
 
 int dense_panel_width(int n) { return n <= 2048 ? 16 : 8; }
 
-int solver_occupancy(int n, int fp32, size_t dyn_lds, int *per_cu)
+int solver_occupancy(int n, int fp32, int refine, size_t dyn_lds, int *per_cu)
 {
-    const void *k = pick_kernel(n, fp32);
+    const void *k = pick_kernel(n, fp32, refine);
     if (dyn_lds > 64 * 1024 &&
         hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds) != hipSuccess)
         return -1;
@@ -1067,7 +1325,7 @@ hipError_t launch_solve(const SolveArgs &args, size_t dyn_lds, hipStream_t strea
 {
     SolveArgs a = args;
     void *params[] = {&a};
-    return hipLaunchCooperativeKernel(pick_kernel(a.m.S, a.fp32), dim3(a.G), dim3(NT), params,
+    return hipLaunchCooperativeKernel(pick_kernel(a.m.S, a.fp32, a.refine), dim3(a.G), dim3(NT), params,
                                       (unsigned int)dyn_lds, stream);
 }
 

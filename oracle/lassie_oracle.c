@@ -47,7 +47,11 @@
  *     persistent BDF step doubles, capped at be_step (the reference only ever
  *     shrinks it, Simulation.cuh:872, 907);
  *   max_rkf_retries > 0: that many consecutive rejected RKF attempts send the
- *     next rejection to BDF (the reference can cycle on rejections forever).
+ *     next rejection to BDF (the reference can cycle on rejections forever);
+ *   refine: the precision form of the Newton linear solve (orc_refine_solve):
+ *     FP32 factors of (float)(I - hJ), FP64 iterative refinement, FP64 LU of
+ *     the step's matrix when the FP32 LU meets a zero pivot or refinement
+ *     does not converge.
  */
 #include <math.h>
 #include <stdint.h>
@@ -198,6 +202,49 @@ void orc_lu_solve(int n, const double *LU, const int32_t *piv, double *b)
     }
 }
 
+/* Precision form of the Newton linear solve (extends linearSolverLU,
+ * Simulation.cuh:1280-1326): M x = rhs with the FP32 factors LU32/piv of
+ * (float)M and FP64 iterative refinement against the FP64 matrix M
+ * (row-major, unfactored).  Sweep: b32 = (float)r; b32 = LU32 \ b32;
+ * x = x + (double)b32; r_i = rhs_i - sum_j M_ij x_j and
+ * mag_i = |rhs_i| + sum_j |M_ij||x_j|, both as fma chains over j ascending
+ * from 0, then mag_i = mag_i + floor_i when floor is given.  Converged when
+ * |r_i| <= tol * mag_i for every i: the Oettli-Prager componentwise backward
+ * error, with the species' absolute tolerance as a floor in the integrator,
+ * so that components far below it (1e-40 and less, under FP32's normal
+ * range) do not hold refinement back.  x starts at 0 and r at rhs.  Returns
+ * 1 on convergence within max_it sweeps, else 0; *iters counts the sweeps. */
+int orc_refine_solve(int n, const double *M, const float *LU32, const int32_t *piv, const double *rhs,
+                     const double *floor, double *x, double *r, float *b32, double tol, int max_it,
+                     int64_t *iters)
+{
+    for (int i = 0; i < n; i++) {
+        x[i] = 0.0;
+        r[i] = rhs[i];
+    }
+    for (int it = 0; it < max_it; it++) {
+        (*iters)++;
+        for (int i = 0; i < n; i++) b32[i] = (float)r[i];
+        orc32_lu_solve(n, LU32, piv, b32);
+        for (int i = 0; i < n; i++) x[i] = x[i] + (double)b32[i];
+        int bad = 0;
+        for (int i = 0; i < n; i++) {
+            const double *row = M + (size_t)i * n;
+            double pa = 0.0, pm = 0.0;
+            for (int j = 0; j < n; j++) {
+                pa = fma(row[j], x[j], pa);
+                pm = fma(fabs(row[j]), fabs(x[j]), pm);
+            }
+            r[i] = rhs[i] - pa;
+            double mag = fabs(rhs[i]) + pm;
+            if (floor) mag = mag + floor[i];
+            if (!(fabs(r[i]) <= tol * mag)) bad = 1;
+        }
+        if (!bad) return 1;
+    }
+    return 0;
+}
+
 /* Newton norm squared: Thrust transform_reduce with a float-returning square
  * (Simulation.cuh:58-64, 883-888) in a fixed order. */
 double orc_normsq(int n, const double *d)
@@ -337,13 +384,25 @@ static double bdf_step(const orc_model *m, const orc_opts *o, double t, double t
     double errN = 1000;
     int luFail = 0;
     int info = 0;
+    /* precision form: FP32 factors of (float)M until refinement fails to
+     * converge (or the FP32 LU meets a zero pivot); then the FP64 LU of M
+     * for the rest of the step */
+    int fp64 = !o->refine;
     while (itN < o->newton_iter && errN > o->newton_tol) {
         orc_rhs(m, w->yv, w->x);
         for (int s = 0; s < S; s++) w->x[s] = -(w->yv[s] - y1[s] - dt * w->x[s]);
         if (itN == 0) {
             orc_jac(m, w->yv, w->J);
             orc_newton_matrix(S, dt, w->J);
-            info = orc_lu(S, w->J, w->piv);
+            if (!fp64) {
+                const size_t nn = (size_t)S * S;
+                for (size_t e = 0; e < nn; e++) w->J32[e] = (float)w->J[e];
+                if (orc32_lu(S, w->J32, w->piv) != 0) {
+                    fp64 = 1;
+                    st->refine_fallbacks++;
+                }
+            }
+            if (fp64) info = orc_lu(S, w->J, w->piv);
         }
         if (info != 0) {
             orc_repair(S, w->yv, y1);
@@ -352,7 +411,24 @@ static double bdf_step(const orc_model *m, const orc_opts *o, double t, double t
             st->lu_failures++;
             break;
         }
-        orc_lu_solve(S, w->J, w->piv, w->x);
+        if (fp64) {
+            orc_lu_solve(S, w->J, w->piv, w->x);
+        } else if (orc_refine_solve(S, w->J, w->J32, w->piv, w->x, m->atol, w->xr, w->r, w->b32,
+                                    o->refine_tol, o->refine_max_iter, &st->refine_iters)) {
+            memcpy(w->x, w->xr, sizeof(double) * S);
+        } else {
+            fp64 = 1;
+            st->refine_fallbacks++;
+            info = orc_lu(S, w->J, w->piv);
+            if (info != 0) {
+                orc_repair(S, w->yv, y1);
+                *newdt = dt * 0.1;
+                luFail = 1;
+                st->lu_failures++;
+                break;
+            }
+            orc_lu_solve(S, w->J, w->piv, w->x);
+        }
         for (int s = 0; s < S; s++) {
             w->yv1[s] = w->yv[s];
             if (m->feed[s] == 0) w->yv[s] = w->yv[s] + w->x[s];
@@ -419,6 +495,15 @@ int orc_solve_rec(const orc_model *m, const orc_opts *o, const double *tv, int n
     bw.yv = buf + 11 * S;
     bw.yv1 = buf + 12 * S;
     bw.piv = (int32_t *)malloc(sizeof(int32_t) * S);
+    bw.J32 = NULL;
+    bw.b32 = NULL;
+    bw.xr = bw.r = NULL;
+    if (o->refine) {
+        bw.J32 = (float *)malloc(sizeof(float) * (size_t)S * S);
+        bw.b32 = (float *)malloc(sizeof(float) * S);
+        bw.xr = (double *)malloc(sizeof(double) * 2 * (size_t)S);
+        bw.r = bw.xr + S;
+    }
     memcpy(y1, m->x0, sizeof(double) * S);
 
     int i = 0;
@@ -477,5 +562,8 @@ int orc_solve_rec(const orc_model *m, const orc_opts *o, const double *tv, int n
     free(buf);
     free(bw.J);
     free(bw.piv);
+    free(bw.J32);
+    free(bw.b32);
+    free(bw.xr);
     return rc;
 }

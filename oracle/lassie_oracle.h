@@ -29,12 +29,16 @@ typedef struct {
     int32_t max_rkf_retries; /* extension: >0 = this many rejections in a row force BDF */
     int32_t pad_;
     int64_t max_iters;    /* safety cap on loop iterations, 0 = none */
+    double refine_tol;    /* precision form: refinement stop, |r_i| <= tol*(|rhs_i| + (|M||x|)_i + atol_i) */
+    int32_t refine;       /* precision form: 1 = FP32 LU + FP64 refinement, 0 = FP64 LU */
+    int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback */
 } orc_opts;
 
 typedef struct {
     int64_t loop_iters, rkf_attempts, rkf_accepts, bdf_steps, newton_iters;
     int64_t lu_failures, nan_repairs, samples;
     double t_final, dt_final, newdt_final;
+    int64_t refine_iters, refine_fallbacks;
 } orc_stats;
 
 /* Record of the state-changing steps of one solve (the discrete map the
@@ -65,6 +69,8 @@ typedef struct {
 typedef struct {
     double *J, *x, *yv, *yv1;
     int32_t *piv;
+    float *J32, *b32;     /* precision form only */
+    double *xr, *r;
 } orc_bdf_ws;
 
 double orc_mypow(double base, int ex);
@@ -73,6 +79,9 @@ void orc_jac(const orc_model *m, const double *x, double *J);
 void orc_newton_matrix(int S, double h, double *J);
 int orc_lu(int n, double *A, int32_t *piv);
 void orc_lu_solve(int n, const double *LU, const int32_t *piv, double *b);
+int orc_refine_solve(int n, const double *M, const float *LU32, const int32_t *piv, const double *rhs,
+                     const double *floor, double *x, double *r, float *b32, double tol, int max_it,
+                     int64_t *iters);
 double orc_normsq(int n, const double *d);
 double orc_min_first(int n, const double *d);
 int orc_rkf_ws_attempt(const orc_model *m, double rtol, double dt, const double *y1, orc_rkf_ws *w);

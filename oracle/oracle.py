@@ -30,7 +30,8 @@ class OrcOpts(C.Structure):
     _fields_ = [("rkf_step", C.c_double), ("be_step", C.c_double), ("newton_tol", C.c_double),
                 ("stiffness_tol", C.c_double), ("rtol", C.c_double), ("newton_iter", C.c_int32),
                 ("bdf_regrow", C.c_int32), ("max_rkf_retries", C.c_int32), ("pad_", C.c_int32),
-                ("max_iters", C.c_int64)]
+                ("max_iters", C.c_int64), ("refine_tol", C.c_double), ("refine", C.c_int32),
+                ("refine_max_iter", C.c_int32)]
 
 
 class OrcStats(C.Structure):
@@ -38,7 +39,8 @@ class OrcStats(C.Structure):
                 ("rkf_accepts", C.c_int64), ("bdf_steps", C.c_int64),
                 ("newton_iters", C.c_int64), ("lu_failures", C.c_int64),
                 ("nan_repairs", C.c_int64), ("samples", C.c_int64),
-                ("t_final", C.c_double), ("dt_final", C.c_double), ("newdt_final", C.c_double)]
+                ("t_final", C.c_double), ("dt_final", C.c_double), ("newdt_final", C.c_double),
+                ("refine_iters", C.c_int64), ("refine_fallbacks", C.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -106,6 +108,9 @@ def lib():
         L.orc32_rkf_attempt.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_float, _f32p, _f32p, _f32p, _f32p]
         L.orc32_rkf_attempt.restype = C.c_int
         L.orc32_solve.argtypes = L.orc_solve.argtypes
+        L.orc_refine_solve.argtypes = [C.c_int, _f64p, _f32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f32p,
+                                       C.c_double, C.c_int, C.POINTER(C.c_int64)]
+        L.orc_refine_solve.restype = C.c_int
         L.orc32_solve.restype = C.c_int
         _lib = L
     return _lib
@@ -138,9 +143,12 @@ class _Bound:
 
 
 def opts(rkf_step=1e-3, be_step=1e-2, newton_tol=1e-6, stiffness_tol=1e-6, rtol=0.0,
-         newton_iter=10000, bdf_regrow=0, max_rkf_retries=0, max_iters=0):
+         newton_iter=10000, bdf_regrow=0, max_rkf_retries=0, max_iters=0, refine=0,
+         refine_tol=1e-14, refine_max_iter=4):
+    """Controller options; refine=1 selects the precision form (FP32 LU of
+    (float)M with FP64 refinement, orc_refine_solve)."""
     return OrcOpts(rkf_step, be_step, newton_tol, stiffness_tol, rtol, newton_iter, bdf_regrow,
-                   max_rkf_retries, 0, max_iters)
+                   max_rkf_retries, 0, max_iters, refine_tol, refine, refine_max_iter)
 
 
 def rhs(m, x):
@@ -179,6 +187,22 @@ def lu_solve(LU, piv, b):
     piv = np.ascontiguousarray(piv, np.int32)
     lib().orc_lu_solve(LU.shape[0], _p(LU, _f64p), _p(piv, _i32p), _p(x, _f64p))
     return x
+
+
+def refine_solve(M, LU32, piv, rhs, tol=1e-14, max_it=4, floor=None):
+    """orc_refine_solve: returns (x, r, converged, sweeps)."""
+    fl = None if floor is None else np.ascontiguousarray(floor, np.float64)
+    M = np.ascontiguousarray(M, np.float64)
+    LU32 = np.ascontiguousarray(LU32, np.float32)
+    piv = np.ascontiguousarray(piv, np.int32)
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    n = M.shape[0]
+    x, r, b32 = np.empty(n), np.empty(n), np.empty(n, np.float32)
+    it = C.c_int64(0)
+    ok = lib().orc_refine_solve(n, _p(M, _f64p), _p(LU32, _f32p), _p(piv, _i32p), _p(rhs, _f64p),
+                                None if fl is None else _p(fl, _f64p), _p(x, _f64p), _p(r, _f64p),
+                                _p(b32, _f32p), tol, max_it, C.byref(it))
+    return x, r, bool(ok), it.value
 
 
 def normsq(d):

@@ -1,0 +1,117 @@
+"""Single-model precision form on the GPU (LASSIE_PREC_FP32_REFINE): the FP64
+integrator with the Newton matrix rounded to FP32 and factored on the grid,
+FP64 iterative refinement of every Newton solve, and the FP64 LU of the
+step's matrix when refinement misses its target.
+
+The bar is the plain path's: bit-exact against the oracle's precision form
+(orc_refine_solve inside orc_solve), counters included, because the
+refinement sweeps are specified operation by operation (FP32 solve, x + d,
+fma-chain residual in column order, componentwise test).  Sizes cover the
+16-column panels (n <= 2048) and the 8-column panels of the 4096-species
+configuration; large models run a capped prefix of the loop.
+"""
+import numpy as np
+import pytest
+
+from lassie_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+COUNTERS = ("loop_iters", "rkf_attempts", "rkf_accepts", "bdf_steps", "newton_iters",
+            "lu_failures", "nan_repairs", "samples", "refine_iters", "refine_fallbacks")
+BENCH = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
+REFINE = 1
+
+
+def run_both(gpu_lib, m, t, cap=0, refine_tol=1e-14, refine_max_iter=4, **opts):
+    print(f"[refine] S={m.S} R={m.R} nt={len(t)} cap={cap} tol={refine_tol} "
+          f"sweeps={refine_max_iter} opts={opts}", flush=True)
+    dm = gpu_lib.DeviceModel(m)
+    out_g, yf_g, st_g, rc_g, _ = dm.solve(t, raise_on_maxiter=False, max_iters=cap,
+                                          precision=gpu_lib.PREC_FP32_REFINE, refine_tol=refine_tol,
+                                          refine_max_iter=refine_max_iter, **opts)
+    dm.close()
+    out_o, st_o, rc_o, _, yf_o = O.solve(m, t, max_iters=cap, refine=REFINE, refine_tol=refine_tol,
+                                         refine_max_iter=refine_max_iter, **opts)
+    return (out_g, yf_g, st_g, rc_g), (out_o, yf_o, st_o, rc_o)
+
+
+def assert_identical(g, o):
+    out_g, yf_g, st_g, rc_g = g
+    out_o, yf_o, st_o, rc_o = o
+    assert (rc_g == 0) == (rc_o == 0), (rc_g, rc_o)
+    for key in COUNTERS:
+        assert st_g[key] == st_o[key], (key, st_g[key], st_o[key])
+    assert st_g["t_final"] == st_o["t_final"]
+    assert st_g["newdt_final"] == st_o["newdt_final"]
+    np.testing.assert_array_equal(yf_g, yf_o)
+    n = int(st_o["samples"])
+    np.testing.assert_array_equal(out_g[:n], out_o[:n])
+
+
+@pytest.mark.parametrize("S,seed,cap", [(17, 21, 0), (48, 3, 0), (100, 7, 60), (300, 7, 40),
+                                        (513, 7, 30)])
+def test_mild_steps_refine_without_fallback(gpu_lib, S, seed, cap):
+    """BDF steps of 1e-3 on stiff models: refinement converges in every
+    Newton solve; whole trajectories for the small sizes, capped prefixes
+    for the others (ragged sizes off the 16-column panel grid)."""
+    m = synth.stiff(S, 2 * S, seed, (1e-4, 1e2))
+    t = synth.t_vector(0, 1, 11)
+    g, o = run_both(gpu_lib, m, t, cap=cap, be_step=1e-3, **BENCH)
+    assert o[2]["refine_iters"] > 0 and o[2]["bdf_steps"] > 0
+    assert_identical(g, o)
+
+
+@pytest.mark.parametrize("S,seed,fallbacks", [(64, 1002, 0), (257, 7, 2)])
+def test_stiff_steps(gpu_lib, S, seed, fallbacks):
+    """k logU[1e-4, 1e4] with the default BDF step; on the 257-species model
+    two Newton matrices are too ill-conditioned for FP32 refinement to reach
+    1e-14, so those steps fall back to the FP64 LU part-way through the
+    Newton loop."""
+    m = synth.stiff(S, 2 * S, seed, (1e-4, 1e4))
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 50, 100), cap=300, **BENCH)
+    assert o[2]["refine_fallbacks"] == fallbacks
+    assert_identical(g, o)
+
+
+def test_every_step_falls_back(gpu_lib):
+    """tol 0 and one sweep: each BDF step runs the FP32 LU, one sweep, then
+    the FP64 LU, and the trajectory equals the FP64 form's."""
+    m = synth.stiff(100, 200, 5, (1e-4, 1e4))
+    t = synth.t_vector(0, 2, 5)
+    g, o = run_both(gpu_lib, m, t, refine_tol=0.0, refine_max_iter=1, **BENCH)
+    assert_identical(g, o)
+    st = o[2]
+    assert st["refine_fallbacks"] == st["bdf_steps"] - st["lu_failures"] > 0
+    out64, _, _, _, _ = O.solve(m, t, **BENCH)
+    np.testing.assert_array_equal(g[0], out64)
+
+
+def test_stiff_1024_prefix(gpu_lib):
+    """The 1024 x 1024 stiff workload (k logU[1e-6, 1e6]), first loop iterations."""
+    m = synth.stiff(1024, 1024, 0)
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=10, newton_iter=100, **BENCH)
+    assert o[2]["bdf_steps"] >= 4
+    assert_identical(g, o)
+
+
+def test_eight_column_panels_2100(gpu_lib):
+    """n > 2048: 8-column panels, 16 rows per thread in the FP32 panel."""
+    m = synth.stiff(2100, 2100, 0)
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=4, newton_iter=100, **BENCH)
+    assert o[2]["bdf_steps"] >= 2
+    assert_identical(g, o)
+
+
+def test_option_checks(gpu_lib):
+    m = synth.stiff(32, 64, 1)
+    dm = gpu_lib.DeviceModel(m)
+    t = synth.t_vector(0, 1, 3)
+    with pytest.raises(gpu_lib.LassieError):
+        dm.solve(t, precision=gpu_lib.PREC_BF16_REFINE)
+    with pytest.raises(gpu_lib.LassieError):
+        dm.solve(t, precision=gpu_lib.PREC_FP32_REFINE, refine_tol=-1.0)
+    out, _, st, rc, _ = dm.solve(t, precision=gpu_lib.PREC_FP32_REFINE, **BENCH)
+    assert rc == 0 and np.isfinite(out).all()
+    dm.close()
