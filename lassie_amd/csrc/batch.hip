@@ -24,8 +24,8 @@ namespace lassie {
 // (Oettli-Prager), so small components are as accurate as large ones.
 // Returns true on convergence; x (double, LDS) holds the result.
 __device__ bool refine_solve(const double *__restrict__ M64, const float *__restrict__ M32, int n, const int *perm_src,
-                             const int *moved_to, const double *rhs, double *x, float *b32, double tol, int max_it,
-                             long long &iters, Shared &sh)
+                             const int *moved_to, const int *pf, const double *rhs, double *x, float *b32, double tol,
+                             int max_it, long long &iters, Shared &sh)
 {
     const int tid = threadIdx.x;
     for (int i = tid; i < n; i += NT) {
@@ -35,7 +35,7 @@ __device__ bool refine_solve(const double *__restrict__ M64, const float *__rest
     __syncthreads();
     for (int it = 0; it < max_it; it++) {
         iters++;
-        lu_solve_wg<kBatchNB, float, true>(M32, n, perm_src, moved_to, b32);
+        lu_solve_wg<kBatchNB, float, true>(M32, n, perm_src, moved_to, b32, pf);
         for (int i = tid; i < n; i += NT) x[i] = x[i] + (double)b32[i];
         __syncthreads();
         // r = rhs - M64 x, one wave per row, lanes over columns (coalesced)
@@ -65,6 +65,8 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
     __shared__ Shared sh;
     __shared__ double s_x[kBatchMaxS];  // solve vector / refined correction
     __shared__ float s_b32[kBatchMaxS];
+    __shared__ int s_pf[kBatchMaxS];    // row order of the current factors (compose_rows)
+    static_assert(kBatchMaxS <= 2 * NT, "lu_solve_wg gathers at most two rows per thread");
     __shared__ int s_idx;
     const int tid = threadIdx.x;
     const int slot = blockIdx.x;
@@ -223,6 +225,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                     }
                 }
                 if (fp64) inf = wg_lu_rm<double, false, RT>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                if (!inf) compose_rows<kBatchNB>(perm_src, moved_to, S, s_pf);
                 st_lu++;
                 {
                     const unsigned long long now = wall_clock64();
@@ -248,17 +251,18 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                     if (fp64) {
                         for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
                         __syncthreads();
-                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x);
-                    } else if (!refine_solve(M, M32, S, perm_src, moved_to, rhs, s_x, s_b32, A.refine_tol,
+                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x, s_pf);
+                    } else if (!refine_solve(M, M32, S, perm_src, moved_to, s_pf, rhs, s_x, s_b32, A.refine_tol,
                                              A.refine_max_iter, st_ref, sh)) {
                         // refinement did not converge: FP64 LU for the rest of this step
                         fp64 = true;
                         st_fb++;
                         inf = wg_lu_rm<double, false, RT>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
                         if (inf) continue;  // handled at the top of the loop as an LU failure
+                        compose_rows<kBatchNB>(perm_src, moved_to, S, s_pf);
                         for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
                         __syncthreads();
-                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x);
+                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x, s_pf);
                     }
                     double part = 0.0;
                     for (int s = tid; s < S; s += NT) {

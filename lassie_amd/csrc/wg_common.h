@@ -592,23 +592,61 @@ __device__ void panel_factor_s(const Store st, int n, int p, int *piv, int *perm
     __syncthreads();
 }
 
+// Final row order of factors left by the panels' row moves (perm_src,
+// moved_to of panel_factor): row i of L U is row pf[i] of M.  One group.
+template <int NBC>
+__device__ void compose_rows(const int *perm_src, const int *moved_to, int n, int *pf)
+{
+    const int tid = threadIdx.x;
+    for (int s = tid; s < n; s += NT) pf[s] = s;
+    __syncthreads();
+    const int P = (n + NBC - 1) / NBC;
+    for (int p = 0; p < P; p++) {
+        const int r0 = p * NBC;
+        const int nbp = min(NBC, n - r0);
+        int v1 = 0, v2 = 0, dst = -1;
+        if (tid < nbp) {
+            v1 = pf[perm_src[r0 + tid]];
+            v2 = pf[r0 + tid];
+            dst = moved_to[r0 + tid];
+        }
+        __syncthreads();
+        if (tid < nbp) {
+            pf[r0 + tid] = v1;
+            if (dst >= r0 + nbp) pf[dst] = v2;
+        }
+        __syncthreads();
+    }
+}
+
 // getrs with one right-hand side (Simulation.cuh:1316), one group, b in LDS.
 // M holds L\U in final row order, so the panels' row moves are applied to b
-// first (equal to LAPACK's sequential interchanges), then forward (k
+// first (equal to LAPACK's sequential interchanges; with pf, the composed
+// order of compose_rows, n <= 2 NT, gathered in one pass), then forward (k
 // ascending) and backward (k descending) substitution as fma chains, in
 // 64-row blocks: the diagonal block by one wave with register shuffles, the
-// rows outside it by all waves with four rows and 16 matrix loads per row in
-// flight.
+// rows outside it by all waves, two rows per thread, the next 16 matrix
+// loads of each row in flight while the current 16 are applied.
 // T: element type of the factors and of b; RM: row-major (else column-major).
 template <int NBC, typename T, bool RM>
-__device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src, const int *moved_to, T *b)
+__device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src, const int *moved_to, T *b,
+                            const int *pf = nullptr)
 {
     auto at = [&](int row, int col) -> T { return RM ? M[(size_t)row * n + col] : M[(size_t)col * n + row]; };
     constexpr int TB = 64;
-    constexpr int RB = 4;
+    constexpr int RB = 2;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int P = (n + NBC - 1) / NBC;
+    const int P = pf ? 0 : (n + NBC - 1) / NBC;
+    if (pf) {
+        T g0 = 0, g1 = 0;
+        if (tid < n) g0 = b[pf[tid]];
+        if (tid + NT < n) g1 = b[pf[tid + NT]];
+        __syncthreads();
+        if (tid < n) b[tid] = g0;
+        if (tid + NT < n) b[tid + NT] = g1;
+        __syncthreads();
+    }
     for (int p = 0; p < P; p++) {
         const int r0 = p * NBC;
         const int nbp = min(NBC, n - r0);
@@ -632,13 +670,17 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
         const int nb = min(TB, n - r0);
         if (tid < 64) {
             T bi = (lane < nb) ? b[r0 + lane] : T(0);
+            T lsub[16], lnx[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) lsub[kk] = (kk < lane && lane < nb) ? at(r0 + lane, r0 + kk) : T(0);
 #pragma unroll
             for (int sb = 0; sb < TB / 16; sb++) {
-                T lsub[16];
+                if (sb + 1 < TB / 16) {
 #pragma unroll
-                for (int kk = 0; kk < 16; kk++) {
-                    const int k = sb * 16 + kk;
-                    lsub[kk] = (k < lane && lane < nb) ? at(r0 + lane, r0 + k) : T(0);
+                    for (int kk = 0; kk < 16; kk++) {
+                        const int k = (sb + 1) * 16 + kk;
+                        lnx[kk] = (k < lane && lane < nb) ? at(r0 + lane, r0 + k) : T(0);
+                    }
                 }
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) {
@@ -648,6 +690,10 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
                         if (lane > k && lane < nb) bi = fma(-lsub[kk], bk, bi);
                     }
                 }
+                if (sb + 1 < TB / 16) {
+#pragma unroll
+                    for (int kk = 0; kk < 16; kk++) lsub[kk] = lnx[kk];
+                }
             }
             if (lane < nb) b[r0 + lane] = bi;
         }
@@ -656,16 +702,23 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
             T acc[RB];
 #pragma unroll
             for (int r = 0; r < RB; r++) acc[r] = (i0 + r * NT < n) ? b[i0 + r * NT] : T(0);
+            T l[RB][16], ln[RB][16];
+#pragma unroll
+            for (int r = 0; r < RB; r++)
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++)
+                    l[r][kk] = (kk < nb && i0 + r * NT < n) ? at(i0 + r * NT, r0 + kk) : T(0);
 #pragma unroll
             for (int sb = 0; sb < TB / 16; sb++) {
-                T l[RB][16];
+                if (sb + 1 < TB / 16) {
 #pragma unroll
-                for (int r = 0; r < RB; r++)
+                    for (int r = 0; r < RB; r++)
 #pragma unroll
-                    for (int kk = 0; kk < 16; kk++) {
-                        const int k = sb * 16 + kk;
-                        l[r][kk] = (k < nb && i0 + r * NT < n) ? at(i0 + r * NT, r0 + k) : T(0);
-                    }
+                        for (int kk = 0; kk < 16; kk++) {
+                            const int k = (sb + 1) * 16 + kk;
+                            ln[r][kk] = (k < nb && i0 + r * NT < n) ? at(i0 + r * NT, r0 + k) : T(0);
+                        }
+                }
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) {
                     const int k = sb * 16 + kk;
@@ -674,6 +727,12 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
 #pragma unroll
                         for (int r = 0; r < RB; r++) acc[r] = fma(-l[r][kk], bk, acc[r]);
                     }
+                }
+                if (sb + 1 < TB / 16) {
+#pragma unroll
+                    for (int r = 0; r < RB; r++)
+#pragma unroll
+                        for (int kk = 0; kk < 16; kk++) l[r][kk] = ln[r][kk];
                 }
             }
 #pragma unroll
@@ -687,13 +746,20 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
         const int nb = min(TB, n - r0);
         if (tid < 64) {
             T bi = (lane < nb) ? b[r0 + lane] : T(0);
+            T usub[16], unx[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) {
+                const int k = (TB / 16 - 1) * 16 + kk;
+                usub[kk] = (k >= lane && k < nb && lane < nb) ? at(r0 + lane, r0 + k) : T(1);
+            }
 #pragma unroll
             for (int sb = TB / 16 - 1; sb >= 0; sb--) {
-                T usub[16];
+                if (sb > 0) {
 #pragma unroll
-                for (int kk = 0; kk < 16; kk++) {
-                    const int k = sb * 16 + kk;
-                    usub[kk] = (k >= lane && k < nb && lane < nb) ? at(r0 + lane, r0 + k) : T(1);
+                    for (int kk = 0; kk < 16; kk++) {
+                        const int k = (sb - 1) * 16 + kk;
+                        unx[kk] = (k >= lane && k < nb && lane < nb) ? at(r0 + lane, r0 + k) : T(1);
+                    }
                 }
 #pragma unroll
                 for (int kk = 15; kk >= 0; kk--) {
@@ -704,6 +770,10 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
                         if (lane < k) bi = fma(-usub[kk], xk, bi);
                     }
                 }
+                if (sb > 0) {
+#pragma unroll
+                    for (int kk = 0; kk < 16; kk++) usub[kk] = unx[kk];
+                }
             }
             if (lane < nb) b[r0 + lane] = bi;
         }
@@ -712,16 +782,25 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
             T acc[RB];
 #pragma unroll
             for (int r = 0; r < RB; r++) acc[r] = (i0 + r * NT < r0) ? b[i0 + r * NT] : T(0);
+            T u[RB][16], un[RB][16];
+#pragma unroll
+            for (int r = 0; r < RB; r++)
+#pragma unroll
+                for (int kk = 0; kk < 16; kk++) {
+                    const int k = (TB / 16 - 1) * 16 + kk;
+                    u[r][kk] = (k < nb && i0 + r * NT < r0) ? at(i0 + r * NT, r0 + k) : T(0);
+                }
 #pragma unroll
             for (int sb = TB / 16 - 1; sb >= 0; sb--) {
-                T u[RB][16];
+                if (sb > 0) {
 #pragma unroll
-                for (int r = 0; r < RB; r++)
+                    for (int r = 0; r < RB; r++)
 #pragma unroll
-                    for (int kk = 0; kk < 16; kk++) {
-                        const int k = sb * 16 + kk;
-                        u[r][kk] = (k < nb && i0 + r * NT < r0) ? at(i0 + r * NT, r0 + k) : T(0);
-                    }
+                        for (int kk = 0; kk < 16; kk++) {
+                            const int k = (sb - 1) * 16 + kk;
+                            un[r][kk] = (k < nb && i0 + r * NT < r0) ? at(i0 + r * NT, r0 + k) : T(0);
+                        }
+                }
 #pragma unroll
                 for (int kk = 15; kk >= 0; kk--) {
                     const int k = sb * 16 + kk;
@@ -730,6 +809,12 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
 #pragma unroll
                         for (int r = 0; r < RB; r++) acc[r] = fma(-u[r][kk], xk, acc[r]);
                     }
+                }
+                if (sb > 0) {
+#pragma unroll
+                    for (int r = 0; r < RB; r++)
+#pragma unroll
+                        for (int kk = 0; kk < 16; kk++) u[r][kk] = un[r][kk];
                 }
             }
 #pragma unroll
