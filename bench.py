@@ -148,6 +148,38 @@ def differentiation_form(B, t_end):
             "finite_fraction": float(np.mean([np.isfinite(g).all() for g in gks]))}
 
 
+def precision_form_4096(cap):
+    """The 4096 x 4096 configuration (same generator as the headline, seed 0)
+    in the plain FP64 form and in the single-model precision form (FP32 grid
+    LU, FP64 refinement to a componentwise 1e-14 with the species' atol as
+    floor), over the first `cap` loop iterations of the same controller."""
+    import numpy as np
+    from lassie_amd import _lib, synth
+    m = synth.stiff(4096, 4096, 0, WORKLOAD["k_range"])
+    t = synth.t_vector(0.0, WORKLOAD["t_end"], WORKLOAD["n_samples"])
+    dm = _lib.DeviceModel(m)
+    res, finals = {}, {}
+    for name, kw in (("fp64", {}), ("fp32_refine", dict(precision=_lib.PREC_FP32_REFINE))):
+        say(f"  4096x4096 {name}, first {cap} loop iterations")
+        t0 = time.perf_counter()
+        _, yf, st, rc, _ = dm.solve(t, raise_on_maxiter=False, max_iters=cap, **CONTROLLER, **kw)
+        wall = time.perf_counter() - t0
+        finals[name] = yf
+        lu = max(st["lu_factorizations"], 1)
+        res[name] = {"wall_s": wall, "kernel_ms": st["kernel_ms"], "bdf_steps": st["bdf_steps"],
+                     "newton_iters": st["newton_iters"], "lu_factorizations": st["lu_factorizations"],
+                     "lu_ms_per_factorization": st["phase_ms"][2] / lu,
+                     "newton_ms_per_iteration": st["phase_ms"][3] / max(st["newton_iters"], 1),
+                     "refine_sweeps": st["refine_iters"], "refine_fallbacks": st["refine_fallbacks"],
+                     "groups": st["groups"]}
+        say(f"  {name}: {wall:.1f} s")
+    dm.close()
+    a, b = finals["fp64"], finals["fp32_refine"]
+    res["max_rel_err_final_state"] = float(np.max(np.abs(b - a) / (np.abs(a) + m.atol)))
+    res["loop_iterations"] = cap
+    return res
+
+
 def say(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -165,6 +197,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the single-precision loop (-p float)")
     ap.add_argument("--diff-batch", type=int, default=256)
     ap.add_argument("--diff-t-end", type=float, default=0.2)
+    ap.add_argument("--precision-4096", type=int, default=0, metavar="N",
+                    help="also compare the plain and precision forms on the 4096x4096 model over "
+                         "its first N loop iterations (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: solves are "
@@ -255,6 +290,9 @@ def main():
     if not args.no_diff:
         say(f"differentiation form, {args.diff_batch} parameterisations of 512x1024, t in [0, {args.diff_t_end}]")
         line["differentiation_form"] = differentiation_form(args.diff_batch, args.diff_t_end)
+    if args.precision_4096 > 0:
+        say(f"precision form on 4096x4096, {args.precision_4096} loop iterations")
+        line["precision_form_4096"] = precision_form_4096(args.precision_4096)
     print(json.dumps(line), flush=True)
 
 
