@@ -86,7 +86,7 @@ typedef struct {
     int32_t max_rkf_retries; /* extension: N rejections in a row force BDF; default 0 = off */
     int32_t precision;     /* LASSIE_PREC_*, default LASSIE_PREC_FP64 */
     int64_t max_iters;     /* cap on loop iterations; default 100000000 */
-    double refine_tol;     /* refinement stop: |r_i| <= tol*((|M||x|)_i + |b_i|) for all i, default 1e-14 */
+    double refine_tol;     /* refinement stop: |r_i| <= tol*((|M||x|)_i + |b_i| + atol_i) for all i, default 1e-14 */
     int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback, default 4 */
     int32_t reserved;      /* diagnostics: bit 1 keeps the control group's state in global memory */
 } lassie_opts;

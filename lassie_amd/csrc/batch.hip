@@ -20,12 +20,14 @@ namespace lassie {
 
 // Mixed-precision Newton correction: solve M d = rhs with the FP32 factors and
 // FP64 residuals r = rhs - M64 x, until every component meets the
-// componentwise backward-error test |r_i| <= tol * ((|M64| |x|)_i + |rhs_i|)
-// (Oettli-Prager), so small components are as accurate as large ones.
-// Returns true on convergence; x (double, LDS) holds the result.
+// componentwise backward-error test |r_i| <= tol * ((|M64| |x|)_i + |rhs_i| + atol_i)
+// (Oettli-Prager with the species' absolute tolerance as floor), so small
+// components are as accurate as large ones, while components far below atol
+// (under FP32's normal range, which no FP32 correction can reach) do not hold
+// refinement back.  Returns true on convergence; x (double, LDS) holds the result.
 __device__ bool refine_solve(const double *__restrict__ M64, const float *__restrict__ M32, int n, const int *perm_src,
-                             const int *moved_to, const int *pf, const double *rhs, double *x, float *b32, double tol,
-                             int max_it, long long &iters, Shared &sh)
+                             const int *moved_to, const int *pf, const double *atol, const double *rhs, double *x,
+                             float *b32, double tol, int max_it, long long &iters, Shared &sh)
 {
     const int tid = threadIdx.x;
     for (int i = tid; i < n; i += NT) {
@@ -50,7 +52,7 @@ __device__ bool refine_solve(const double *__restrict__ M64, const float *__rest
                 pm = fma(fabs(mv), fabs(xv), pm);
             }
             const double acc = rhs[i] - wave_sum(pa);
-            const double mag = fabs(rhs[i]) + wave_sum(pm);
+            const double mag = (fabs(rhs[i]) + wave_sum(pm)) + atol[i];
             if (lane == 0) b32[i] = (float)acc;
             bad |= !(fabs(acc) <= tol * mag);
         }
@@ -252,7 +254,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                         for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
                         __syncthreads();
                         lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x, s_pf);
-                    } else if (!refine_solve(M, M32, S, perm_src, moved_to, s_pf, rhs, s_x, s_b32, A.refine_tol,
+                    } else if (!refine_solve(M, M32, S, perm_src, moved_to, s_pf, m.atol, rhs, s_x, s_b32, A.refine_tol,
                                              A.refine_max_iter, st_ref, sh)) {
                         // refinement did not converge: FP64 LU for the rest of this step
                         fp64 = true;

@@ -2,9 +2,10 @@
 
 FP64: every instance of the batch must equal the oracle bit for bit (same
 algorithm and operation order as the single-model path).  FP32 / BF16 LU with
-FP64 refinement: the relative error against the FP64 form,
-|a - b| / (|b| + atol), and the refinement work are reported (the loop's
-accept/reject decisions are discontinuous, so no bound holds for every model).
+FP64 refinement: on well-conditioned Newton matrices, the FP64 form's
+decisions and a 1e-4 bound on |a - b| / (|b| + atol); on stiff instances the
+relative error and the refinement work are reported (the loop's accept/reject
+decisions are discontinuous, so no bound holds for every model).
 The batched plan (graph replay, autotuned variant) equals the batched form.
 """
 import dataclasses
@@ -59,16 +60,42 @@ def test_batch_single_model_agrees_with_single_path(gpu_lib):
 
 
 @pytest.mark.parametrize("prec", [1, 2], ids=["fp32", "bf16"])
-def test_batch_mixed_precision(gpu_lib, prec):
-    """FP32 / BF16 factors with FP64 refinement on uneven stiff instances.
+def test_batch_mixed_precision_mild_steps(gpu_lib, prec):
+    """FP32 / BF16 factors with FP64 refinement where the Newton matrices are
+    well conditioned (k logU[1e-4, 1e2], BDF step 1e-3, t in [0, 1]):
+    refinement meets its componentwise target (floor: the species' atol)
+    in every solve, the controller takes the FP64 form's decisions, and
+    every instance stays within 1e-4 of it, |a - b| / (|b| + atol) (6.2e-5 at
+    worst measured: the step-size feedback of the RKF45 controller amplifies
+    1e-16 differences of the corrections)."""
+    models = [synth.stiff(S, 2 * S, 7000 + i, (1e-4, 1e2)) for i, S in enumerate((16, 48, 100, 200, 333, 512))]
+    t = synth.t_vector(0, 1, 11)
+    kw = dict(be_step=1e-3, max_iters=2_000_000, **BENCH)
+    print(f"[batch] precision {prec} mild sizes {[m.S for m in models]}", flush=True)
+    dms = [gpu_lib.DeviceModel(m) for m in models]
+    ref, _, ref_st, _, rc0 = gpu_lib.solve_batch(dms, t, **kw)
+    outs, _, stats, _, rc = gpu_lib.solve_batch(dms, t, precision=prec, refine_max_iter=10, **kw)
+    assert rc0 == 0 and rc == 0
+    for m, a, b, st, rs in zip(models, outs, ref, stats, ref_st):
+        rel = float((np.abs(a - b) / (np.abs(b) + m.atol[None, :])).max())
+        print(f"[batch] S={m.S} rel={rel:.2e} refine={st['refine_iters']} "
+              f"fallbacks={st['refine_fallbacks']} bdf={st['bdf_steps']}", flush=True)
+        assert st["refine_iters"] >= st["newton_iters"] > 0 and st["refine_fallbacks"] == 0
+        for k in ("loop_iters", "bdf_steps", "newton_iters", "nan_repairs"):
+            assert st[k] == rs[k], k
+        assert rel <= 1e-4
+    for dm in dms:
+        dm.close()
 
-    Where the refinement meets its componentwise backward-error target the
-    Newton corrections equal the FP64 ones to rounding.  Checked: every
-    instance completes with finite samples, refinement work is reported, and
-    at least one instance follows the FP64 form's decision sequence exactly.
-    The relative error |a - b| / (|b| + atol) against the FP64 form is printed
-    per instance (not asserted: on these stiff models a near-zero species can
-    change sign between two accurate solves and flip a repair decision).
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["fp32", "bf16"])
+def test_batch_mixed_precision_stiff(gpu_lib, prec):
+    """The same form on uneven stiff instances (k logU[1e-4, 1e4], t in
+    [0, 50]): every instance completes with finite samples and refinement
+    work is reported.  The relative error against the FP64 form is printed,
+    not asserted: these models' accept/reject and repair decisions are
+    discontinuous, so corrections that differ at 1e-14 lead to different
+    step sequences (both forms' solves meet a componentwise 1e-14 target).
     """
     models = _batch(3000, 10, (16, 32, 64, 128))
     t = synth.t_vector(0, 50, 100)
@@ -77,20 +104,12 @@ def test_batch_mixed_precision(gpu_lib, prec):
     ref, _, ref_st, _, rc0 = gpu_lib.solve_batch(dms, t, max_iters=2_000_000, **BENCH)
     outs, _, stats, _, rc = gpu_lib.solve_batch(dms, t, max_iters=2_000_000, precision=prec, **BENCH)
     assert rc0 == 0 and rc == 0
-    same = 0
-    worst_same = 0.0
     for m, a, b, st, rs in zip(models, outs, ref, stats, ref_st):
         assert st["samples"] == t.size and np.all(np.isfinite(a))
         assert st["refine_iters"] > 0
         rel = float((np.abs(a - b) / (np.abs(b) + m.atol[None, :])).max())
-        if all(st[k] == rs[k] for k in ("loop_iters", "bdf_steps", "newton_iters", "nan_repairs")):
-            same += 1
-            worst_same = max(worst_same, rel)
         print(f"[batch] S={m.S} rel={rel:.2e} refine={st['refine_iters']} "
               f"fallbacks={st['refine_fallbacks']}", flush=True)
-    print(f"[batch] {same}/{len(models)} instances on the FP64 decision sequence, "
-          f"max rel err among them {worst_same:.2e}", flush=True)
-    assert same >= 1
     for dm in dms:
         dm.close()
 
