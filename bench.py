@@ -238,7 +238,7 @@ def main():
                    "k_range": list(w["k_range"]), "controller": CONTROLLER},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "solve_kernel<4,16,double> (persistent: one launch = one solve)",
+                     "kernel": "solve_kernel<4,16,double,false> (persistent: one launch = one solve)",
                      "flops_per_launch": fl, "kernel_ms": k_ms,
                      "phase_ms": {"control_rkf": stats["phase_ms"][0],
                                   "jacobian": stats["phase_ms"][1], "lu": stats["phase_ms"][2],
