@@ -95,8 +95,27 @@ def chosen_form(n_batch, n_plain, with_plain):
     plan = _lib.BatchPlan(dms, t, **CONTROLLER)
     variant, tile_rows, trial_ms = plan.info()
     say(f"  plan prepared, variant {variant} ({time.perf_counter() - t0:.1f} s)")
+    # the run blocks this thread; another one reports each time the device's
+    # count of finished instances has grown by 1/64 of the batch
+    import threading
+    finished = threading.Event()
+
+    def follow(t_start):
+        step, last = max(1, n_batch // 64), 0
+        while not finished.wait(2.0):
+            done = plan.progress()
+            if done >= last + step:
+                last = done
+                say(f"  {done} of {n_batch} instances finished ({time.perf_counter() - t_start:.1f} s)")
+
     t0 = time.perf_counter()
-    outs, _, st, ms, rc = plan.run()
+    watcher = threading.Thread(target=follow, args=(t0,), daemon=True)
+    watcher.start()
+    try:
+        outs, _, st, ms, rc = plan.run()
+    finally:
+        finished.set()
+        watcher.join()
     wall = time.perf_counter() - t0
     say(f"  batched run: {wall:.2f} s")
     plan.close()

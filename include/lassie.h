@@ -156,6 +156,13 @@ int lassie_batch_plan_create(lassie_model *const *models, int32_t n_models, cons
 int lassie_batch_plan_run(lassie_batch_plan *plan, double *out, double *y_final, lassie_stats *stats,
                           double *elapsed_ms);
 int lassie_batch_plan_info(const lassie_batch_plan *plan, int32_t *variant, int32_t *tile_rows, float *trial_ms);
+/* Instances finished so far by the run in progress (or by the last run), for
+ * a caller that follows a long lassie_batch_plan_run from another thread:
+ * every workgroup publishes its count to pinned host memory when an instance
+ * ends.  Reading is safe while the run is in flight.  No counterpart in the
+ * reference (it runs one model per process and reports its progress in the
+ * ProgressSimulation file, Simulation.cuh:660-672). */
+int lassie_batch_plan_progress(const lassie_batch_plan *plan, int64_t *done);
 int lassie_batch_plan_destroy(lassie_batch_plan *plan);
 
 /* Replaces the rate constants (c_vector) of an uploaded model, k[R].  A tape

@@ -70,7 +70,7 @@ EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy",
            "lassie_solve_batch", "lassie_model_set_rates", "lassie_forward_batch",
            "lassie_adjoint_batch", "lassie_tape_info", "lassie_tape_read", "lassie_tape_destroy",
            "lassie_batch_plan_create", "lassie_batch_plan_run", "lassie_batch_plan_info",
-           "lassie_batch_plan_destroy",
+           "lassie_batch_plan_progress", "lassie_batch_plan_destroy",
            "lassie_csr_from_dense", "lassie_device_count", "lassie_last_error", "lassie_abi_version")
 
 
@@ -113,6 +113,7 @@ def lib():
                                                C.c_int32, C.POINTER(C.c_void_p)]
         L.lassie_batch_plan_run.argtypes = [C.c_void_p, _f64p, _f64p, C.POINTER(Stats), C.POINTER(C.c_double)]
         L.lassie_batch_plan_info.argtypes = [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_float)]
+        L.lassie_batch_plan_progress.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         L.lassie_batch_plan_destroy.argtypes = [C.c_void_p]
         L.lassie_device_count.argtypes = [_i32p]
         L.lassie_last_error.restype = C.c_char_p
@@ -365,6 +366,12 @@ class BatchPlan:
         ms = (C.c_float * 3)()
         _check(lib().lassie_batch_plan_info(self.h, _p(v, _i32p), _p(rt, _i32p), ms))
         return int(v[0]), int(rt[0]), list(ms)
+
+    def progress(self):
+        """Instances finished by the run in flight (safe from another thread)."""
+        done = C.c_int64(0)
+        _check(lib().lassie_batch_plan_progress(self.h, C.byref(done)))
+        return int(done.value)
 
     def close(self):
         if getattr(self, "h", None):

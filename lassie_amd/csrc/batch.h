@@ -54,6 +54,9 @@ struct BatchArgs {
     int *tape_sample;  // nt per instance
     double *tape_ckpt;
     long long tape_cap;
+    // per slot: instances it has finished, in pinned host memory, so that the
+    // host can follow a long launch (null: not kept)
+    unsigned int *done_host;
 };
 
 // Kernel variants (row tiles of the LU trailing update in flight per wave):

@@ -80,6 +80,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
     int *piv = wi, *perm_src = wi + smax, *moved_to = wi + 2 * smax, *info = wi + 3 * smax;
     int *rowcnt = wi + 3 * smax + 16, *rows = rowcnt + Pmax;
 
+    unsigned int n_done = 0;  // instances this slot has finished
     while (true) {
         if (tid == 0) s_idx = atomicAdd(A.next, 1);
         __syncthreads();
@@ -332,6 +333,10 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
             d[0] = t;
             d[1] = dt;
             d[2] = newdt;
+            if (A.done_host) {
+                n_done++;
+                __hip_atomic_store(A.done_host + slot, n_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         }
         __syncthreads();
     }

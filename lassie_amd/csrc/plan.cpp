@@ -41,6 +41,7 @@ struct lassie_batch_plan {
     long long *d_st = nullptr;
     double *h_out = nullptr, *h_y = nullptr, *h_dst = nullptr;
     long long *h_st = nullptr;
+    unsigned int *h_done = nullptr;  // per slot: instances finished in the current run
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
@@ -57,7 +58,7 @@ static void free_plan(lassie_batch_plan *p)
                      p->d_y,    p->d_ws64,  p->d_dst,  p->d_ws32, p->d_st};
     for (void *q : dptrs)
         if (q) (void)hipFree(q);
-    void *hptrs[] = {p->h_out, p->h_y, p->h_dst, p->h_st};
+    void *hptrs[] = {p->h_out, p->h_y, p->h_dst, p->h_st, p->h_done};
     for (void *q : hptrs)
         if (q) (void)hipHostFree(q);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -135,7 +136,8 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         (e = hipHostMalloc((void **)&P->h_out, sizeof(double) * P->out_tot)) != hipSuccess ||
         (e = hipHostMalloc((void **)&P->h_y, sizeof(double) * P->y_tot)) != hipSuccess ||
         (e = hipHostMalloc((void **)&P->h_dst, sizeof(double) * 4 * n_models)) != hipSuccess ||
-        (e = hipHostMalloc((void **)&P->h_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess)
+        (e = hipHostMalloc((void **)&P->h_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&P->h_done, sizeof(unsigned int) * slots)) != hipSuccess)
         return bail(LASSIE_ERR_NOMEM, std::string("batch plan: ") + hipGetErrorString(e));
     if ((e = hipMemcpy(P->d_inst, inst.data(), sizeof(lassie::BatchInst) * n_models, hipMemcpyHostToDevice)) !=
             hipSuccess ||
@@ -173,6 +175,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     a.ws32_stride = ws32_stride;
     a.wsi = P->d_wsi;
     a.wsi_stride = wsi_stride;
+    std::memset(P->h_done, 0, sizeof(unsigned int) * slots);
 
     // autotuner: a short trial (64 loop iterations per instance) of every
     // variant; the fastest is kept for this (device, largest S, count, precision)
@@ -186,6 +189,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     if (chosen < 0) {
         lassie::BatchArgs trial = a;
         trial.max_iters = 64;
+        trial.done_host = nullptr;
         float best = 0.f;
         for (int v = 0; v < lassie::kBatchVariants; v++) {
             if ((e = lassie::launch_batch_reset(P->d_next, P->stream)) != hipSuccess ||
@@ -206,6 +210,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         g_tuned[key] = chosen;
     }
     P->variant = chosen;
+    a.done_host = P->h_done;
 
     // the launch sequence as a graph (no memset nodes: the queue head is reset by a kernel)
     if ((e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
@@ -236,6 +241,8 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
     using lassie::api_fail;
     if (!P || !out) return api_fail(LASSIE_ERR_INVALID, "plan and out are required");
     hipError_t e;
+    // no launch of this plan is in flight here (runs are synchronous)
+    std::memset(P->h_done, 0, sizeof(unsigned int) * P->slots);
     if ((e = hipSetDevice(P->device)) != hipSuccess || (e = hipEventRecord(P->ev0, P->stream)) != hipSuccess ||
         (e = hipGraphLaunch(P->exec, P->stream)) != hipSuccess || (e = hipEventRecord(P->ev1, P->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(P->stream)) != hipSuccess)
@@ -252,6 +259,17 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
         maxit |= c[9] == 1;
     }
     if (maxit) return api_fail(LASSIE_ERR_MAXITER, "max_iters reached before the last sample in at least one instance");
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_batch_plan_progress(const lassie_batch_plan *P, int64_t *done)
+{
+    using lassie::api_fail;
+    if (!P || !done) return api_fail(LASSIE_ERR_INVALID, "plan and done are required");
+    int64_t sum = 0;
+    for (int s = 0; s < P->slots; s++)
+        sum += __atomic_load_n(P->h_done + s, __ATOMIC_RELAXED);
+    *done = sum;
     return LASSIE_OK;
 }
 

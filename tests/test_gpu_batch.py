@@ -117,7 +117,7 @@ def test_batch_mixed_precision_stiff(gpu_lib, prec):
 def test_batch_plan_replays_the_batched_form(gpu_lib):
     """lassie_batch_plan_*: the graph replay gives the batched form's results
     bit for bit, twice in a row, and picks up new rates; the autotuner reports
-    its choice."""
+    its choice; the finished-instance count covers every instance after each run."""
     models = synth.uneven_batch(10, 4242, sizes=(16, 48, 100, 200))
     t = synth.t_vector(0, 2, 9)
     kw = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
@@ -127,9 +127,11 @@ def test_batch_plan_replays_the_batched_form(gpu_lib):
     plan = _lib.BatchPlan(dms, t, **kw)
     variant, rt, trial = plan.info()
     assert variant in (0, 1, 2) and rt in (1, 2, 4)
+    assert plan.progress() == 0
     for _ in range(2):
         outs, ys, st, ms, rc = plan.run()
         assert rc == 0 and ms > 0
+        assert plan.progress() == len(models)
         for a, b in zip(outs, ref):
             assert np.array_equal(a, b)
         assert [s["bdf_steps"] for s in st] == [s["bdf_steps"] for s in ref_st]
