@@ -88,6 +88,17 @@ def test_every_step_falls_back(gpu_lib):
     np.testing.assert_array_equal(g[0], out64)
 
 
+def test_singular_newton_matrix(gpu_lib):
+    """A -> 2A with k = 1/h: 1 - hk is 0, so the FP32 LU meets a zero pivot,
+    the step switches to the FP64 LU, which meets it too, and the step is an
+    LU failure (state repaired, BDF step shrunk)."""
+    from tests import cases
+    m = cases.autocatalytic_singular(h=0.01)
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 5, 21), cap=200_000, be_step=0.01, **BENCH)
+    assert o[2]["lu_failures"] >= 1 and o[2]["refine_fallbacks"] >= 1
+    assert_identical(g, o)
+
+
 def test_stiff_1024_prefix(gpu_lib):
     """The 1024 x 1024 stiff workload (k logU[1e-6, 1e6]), first loop iterations."""
     m = synth.stiff(1024, 1024, 0)
