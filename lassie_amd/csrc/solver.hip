@@ -677,34 +677,6 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
 // ---------------------------------------------------------------------------
 // Precision form, helpers (FP64 loop; REF instantiation of solve_kernel).
 
-// Final row order of the factors (row i of L U is row perm_full[i] of M): the
-// panels' row moves composed.  One group.
-template <int NBC>
-__device__ void compose_perm(const SolveArgs &A, int n)
-{
-    const int tid = threadIdx.x;
-    int *pf = A.perm_full;
-    for (int s = tid; s < n; s += NT) pf[s] = s;
-    __syncthreads();
-    const int P = (n + NBC - 1) / NBC;
-    for (int p = 0; p < P; p++) {
-        const int r0 = p * NBC;
-        const int nbp = min(NBC, n - r0);
-        int v1 = 0, v2 = 0, dst = -1;
-        if (tid < nbp) {
-            v1 = pf[A.perm_src[r0 + tid]];
-            v2 = pf[r0 + tid];
-            dst = A.moved_to[r0 + tid];
-        }
-        __syncthreads();
-        if (tid < nbp) {
-            pf[r0 + tid] = v1;
-            if (dst >= r0 + nbp) pf[dst] = v2;
-        }
-        __syncthreads();
-    }
-}
-
 // One refinement sweep after the FP32 solve d = tri32[2S..3S): x_new = x_old
 // + (double)d (x_old = 0 on the first sweep), r = rhs - M x_new and the
 // Oettli-Prager test |r_i| <= tol * (|rhs_i| + (|M||x_new|)_i + atol_i), both
@@ -1037,7 +1009,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 A32.M = reinterpret_cast<double *>(A.M32);
                 if (wg == 0) {
                     st_lu++;
-                    if (!singular) compose_perm<NBC>(A, n);
+                    if (!singular) compose_rows<NBC>(A.perm_src, A.moved_to, n, A.perm_full);
                 }
                 int itN = 0;
                 T errN = 1000;
@@ -1098,7 +1070,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                                 }
                                 break;
                             }
-                            if (wg == 0) compose_perm<NBC>(A, n);
+                            if (wg == 0) compose_rows<NBC>(A.perm_src, A.moved_to, n, A.perm_full);
                             if (!grid_barrier(A.gs, G, gen_local, &ctl->abort)) return;
                         }
                     }
@@ -1134,29 +1106,8 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             const int nblk = (n + kTriBlock - 1) / kTriBlock;
             if (wg == 0) {
                 st_lu++;
-                if (!singular) {
-                    // final row order of the factors: the panels' row moves composed
-                    int *pf = A.perm_full;
-                    for (int s = tid; s < n; s += NT) pf[s] = s;
-                    __syncthreads();
-                    const int P = (n + NBC - 1) / NBC;
-                    for (int p = 0; p < P; p++) {
-                        const int r0 = p * NBC;
-                        const int nbp = min(NBC, n - r0);
-                        int v1 = 0, v2 = 0, dst = -1;
-                        if (tid < nbp) {
-                            v1 = pf[A.perm_src[r0 + tid]];
-                            v2 = pf[r0 + tid];
-                            dst = A.moved_to[r0 + tid];
-                        }
-                        __syncthreads();
-                        if (tid < nbp) {
-                            pf[r0 + tid] = v1;
-                            if (dst >= r0 + nbp) pf[dst] = v2;
-                        }
-                        __syncthreads();
-                    }
-                }
+                // final row order of the factors: the panels' row moves composed
+                if (!singular) compose_rows<NBC>(A.perm_src, A.moved_to, n, A.perm_full);
             }
             if (wg < nblk) {
                 __shared__ int s_cont;
