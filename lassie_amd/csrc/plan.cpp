@@ -137,7 +137,9 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         (e = hipHostMalloc((void **)&P->h_y, sizeof(double) * P->y_tot)) != hipSuccess ||
         (e = hipHostMalloc((void **)&P->h_dst, sizeof(double) * 4 * n_models)) != hipSuccess ||
         (e = hipHostMalloc((void **)&P->h_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess ||
-        (e = hipHostMalloc((void **)&P->h_done, sizeof(unsigned int) * slots)) != hipSuccess)
+        // coherent: the kernel's system-scope stores reach the host while it runs
+        (e = hipHostMalloc((void **)&P->h_done, sizeof(unsigned int) * slots, hipHostMallocCoherent)) !=
+            hipSuccess)
         return bail(LASSIE_ERR_NOMEM, std::string("batch plan: ") + hipGetErrorString(e));
     if ((e = hipMemcpy(P->d_inst, inst.data(), sizeof(lassie::BatchInst) * n_models, hipMemcpyHostToDevice)) !=
             hipSuccess ||
