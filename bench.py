@@ -79,7 +79,49 @@ def cpu_same_algorithm(m, st):
     return proj, dict(lu_s=t_lu, newton_iter_s=t_sol, rkf_attempt_s=t_rkf)
 
 
-def chosen_form(n_batch, n_plain, with_plain):
+def cpu_batched(models, t, st, sample):
+    """CPU baselines of the batched form, on one core (this process has
+    initialised the GPU, so it starts no worker processes): LSODA (the
+    reference's CPU method, scipy odeint) on the first `sample` instances, and
+    the oracle's per-operation cost for each size times the GPU run's
+    per-instance operation counts (the same algorithm)."""
+    import numpy as np
+    from oracle import lsoda
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    for m in models[:sample]:
+        lsoda.solve(m, t, rtol=1e-6)
+    wall = time.perf_counter() - t0
+    unit = {}
+    for m in models:
+        if m.S in unit:
+            continue
+        x = np.asarray(m.x0, float)
+        t0 = time.perf_counter()
+        O.rkf_attempt(m, x, 1e-3, rtol=1e-6)
+        t_rkf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        LU, piv, _ = O.lu(O.newton_matrix(O.jac(m, x), 0.01))
+        t_lu = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(3):
+            O.lu_solve(LU, piv, x)
+            O.rhs(m, x)
+        unit[m.S] = (t_lu, (time.perf_counter() - t0) / 3, t_rkf)
+    proj = sum(d["bdf_steps"] * unit[m.S][0] + d["newton_iters"] * unit[m.S][1]
+               + d["rkf_attempts"] * unit[m.S][2] for m, d in zip(models, st))
+    return {"lsoda": {"value": sample / wall, "unit": "solves/s", "cores": 1, "kind": "port",
+                      "sample": f"the first {sample} of the {len(models)} instances, LSODA (scipy odeint "
+                                f"restating CPU-based LSODA/LSODA.py; rtol 1e-6, atol 1e-12), one core; "
+                                f"independent instances, so N cores give N times this",
+                      "seconds": wall},
+            "same_algorithm_projection": {
+                "value": len(models) / proj, "unit": "solves/s", "cores": 1, "kind": "port",
+                "sample": "oracle per-operation cost per size (one LU, one Newton solve + RHS, one "
+                          "RKF45 attempt) x the GPU run's per-instance counts", "projected_seconds": proj}}
+
+
+def chosen_form(n_batch, n_plain, with_plain, with_cpu=False):
     """Batched FP64 form (a plan: graph replay, autotuned kernel variant) vs
     the plain path on the same instances."""
     from lassie_amd import _lib, synth
@@ -141,6 +183,9 @@ def chosen_form(n_batch, n_plain, with_plain):
         res["max_rel_err_vs_plain"] = 0.0
     for dm in dms:
         dm.close()
+    if with_cpu:
+        say("  CPU baselines of the batched form (one core)")
+        res["cpu_baseline"] = cpu_batched(models, t, st, min(32, n_batch))
     return res
 
 
@@ -308,7 +353,7 @@ def main():
         say(f"single-precision loop done: {el32:.2f} s")
     if not args.no_chosen:
         say(f"batched form, {args.batch} uneven instances; plain path on {args.plain_subsample}")
-        line["chosen_form"] = chosen_form(args.batch, args.plain_subsample, True)
+        line["chosen_form"] = chosen_form(args.batch, args.plain_subsample, True, not args.no_cpu)
     if not args.no_diff:
         say(f"differentiation form, {args.diff_batch} parameterisations of 512x1024, t in [0, {args.diff_t_end}]")
         line["differentiation_form"] = differentiation_form(args.diff_batch, args.diff_t_end)
