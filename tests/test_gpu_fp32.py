@@ -101,6 +101,14 @@ def test_stiff_1024_prefix_fp32(gpu_lib):
     assert_identical(g, o)
 
 
+def test_eight_column_panels_2100_prefix_fp32(gpu_lib):
+    """n > 2048: 8-column float panels, 16 rows per thread."""
+    m = synth.stiff(2100, 2100, 0)
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=4, newton_iter=100, **BENCH)
+    assert o[2]["bdf_steps"] >= 2
+    assert_identical(g, o)
+
+
 def test_fp64_unchanged_by_fp32_tables(gpu_lib):
     """A float solve, then a double solve on the same handle: the double solve
     still matches the FP64 oracle (the float coefficient tables are separate)."""

@@ -104,3 +104,12 @@ def test_stiff_1024_prefix(gpu_lib):
     g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=10, newton_iter=100, **BENCH)
     assert o[2]["bdf_steps"] >= 4
     assert_identical(g, o)
+
+
+def test_eight_column_panels_2100_prefix(gpu_lib):
+    """n > 2048: the FP64 kernel with 8-column panels and the vector strip
+    update (the 4096-species configuration's kernel), first loop iterations."""
+    m = synth.stiff(2100, 2100, 0)
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=4, newton_iter=100, **BENCH)
+    assert o[2]["bdf_steps"] >= 2
+    assert_identical(g, o)
