@@ -467,6 +467,8 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
             m->d_M32 = nullptr;
             return fail(LASSIE_ERR_NOMEM, "FP32 Newton matrix");
         }
+    }
+    if (refine && !m->d_rvec) {
         if (hipMalloc((void **)&m->d_rvec, sizeof(double) * 4 * (size_t)m->S) != hipSuccess) {
             m->d_rvec = nullptr;
             return fail(LASSIE_ERR_NOMEM, "refinement vectors");
