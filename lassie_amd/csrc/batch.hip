@@ -68,6 +68,9 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
     __shared__ double s_x[kBatchMaxS];  // solve vector / refined correction
     __shared__ float s_b32[kBatchMaxS];
     __shared__ int s_pf[kBatchMaxS];    // row order of the current factors (compose_rows)
+    // the instance's state vectors (y1, two stage vectors, k1..k5, w1, yv, rhs):
+    // LDS, so the RKF45 stages and Newton updates make no global round trips
+    __shared__ double s_vec[11 * kBatchMaxS];
     static_assert(kBatchMaxS <= 2 * NT, "lu_solve_wg gathers at most two rows per thread");
     __shared__ int s_idx;
     const int tid = threadIdx.x;
@@ -104,18 +107,18 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
         CtlView v;
         v.recs = I.recs;
         v.hptr = I.hptr;
-        v.y1 = ws;
-        v.ua = ws + S;
-        v.ub = ws + 2 * S;
-        v.k1 = ws + 3 * S;
-        v.k2 = ws + 4 * S;
-        v.k3 = ws + 5 * S;
-        v.k4 = ws + 6 * S;
-        v.k5 = ws + 7 * S;
-        v.w1 = ws + 8 * S;
-        v.yv = ws + 9 * S;
+        v.y1 = s_vec;
+        v.ua = s_vec + S;
+        v.ub = s_vec + 2 * S;
+        v.k1 = s_vec + 3 * S;
+        v.k2 = s_vec + 4 * S;
+        v.k3 = s_vec + 5 * S;
+        v.k4 = s_vec + 6 * S;
+        v.k5 = s_vec + 7 * S;
+        v.w1 = s_vec + 8 * S;
+        v.yv = s_vec + 9 * S;
         v.x = s_x;
-        double *rhs = ws + 10 * S;
+        double *rhs = s_vec + 10 * S;
         double *M = ws + 12 * (size_t)smax;
         double *out = A.out + I.out_off;
 
