@@ -209,8 +209,38 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                 tph = wall_clock64();
                 const size_t nn = (size_t)S * S;
                 for (size_t e = tid; e < nn; e += NT) M[e] = 0.0;
+                // the model arrays the Jacobian walks go to the LU's panel buffer
+                // (idle until the factorization) when they fit, so its chains of
+                // dependent index loads read LDS; the same values, the same order
+                DevModel mj = m;
+                {
+                    const int nh = m.hptr[S], nl = m.lptr[m.R];
+                    const size_t bytes =
+                        8 * (size_t)nh + 4 * ((size_t)(S + 1) + nh + (size_t)(m.R + 1) + 2 * (size_t)nl);
+                    if (bytes <= sizeof(lp_raw)) {
+                        double *hk = lp_raw;
+                        int *hp = reinterpret_cast<int *>(lp_raw + nh);
+                        int *hx = hp + (S + 1), *lp = hx + nh, *ls = lp + (m.R + 1), *le = ls + nl;
+                        for (int e = tid; e < nh; e += NT) {
+                            hk[e] = m.hk[e];
+                            hx[e] = m.hrx[e];
+                        }
+                        for (int e = tid; e <= S; e += NT) hp[e] = m.hptr[e];
+                        for (int e = tid; e <= m.R; e += NT) lp[e] = m.lptr[e];
+                        for (int e = tid; e < nl; e += NT) {
+                            ls[e] = m.lsp[e];
+                            le[e] = m.lexp[e];
+                        }
+                        mj.hk = hk;
+                        mj.hptr = hp;
+                        mj.hrx = hx;
+                        mj.lptr = lp;
+                        mj.lsp = ls;
+                        mj.lexp = le;
+                    }
+                }
                 __syncthreads();
-                for (int r = tid; r < S; r += NT) jac_row<true>(m, v.yv, M, S, r, h);
+                for (int r = tid; r < S; r += NT) jac_row<true>(mj, v.yv, M, S, r, h);
                 __syncthreads();
                 {
                     const unsigned long long now = wall_clock64();
