@@ -42,8 +42,14 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
     for (int p = 0; p < P; p++) {
         const int r0 = p * NBC;
         const int nbp = min(NBC, n - r0);
-        panel_factor<RPT, NBC, T, true, false>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n, rowcnt + p, info,
-                                        sh);
+        // the fewest rows per thread that cover the panel's rows [r0, n): each
+        // row stays with the same thread, so the results do not change
+        if (RPT > 1 && n - r0 <= NT)
+            panel_factor<1, NBC, T, true, false>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n, rowcnt + p,
+                                                 info, sh);
+        else
+            panel_factor<RPT, NBC, T, true, false>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n,
+                                                   rowcnt + p, info, sh);
         __syncthreads();
         const int inf = ld_rlx(info);
         if (inf) return inf;
