@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define LASSIE_ABI_VERSION 1
+#define LASSIE_ABI_VERSION 2
 
 /* Status codes.  The CLI maps them to the reference's exit codes
  * (simulator.cu:61-74, 192-207; Simulation.cuh:38-44, 373-389). */
@@ -89,6 +89,9 @@ typedef struct {
     double refine_tol;     /* refinement stop: |r_i| <= tol*((|M||x|)_i + |b_i| + atol_i) for all i, default 1e-14 */
     int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback, default 4 */
     int32_t reserved;      /* diagnostics: bit 1 keeps the control group's state in global memory */
+    int64_t max_newton;    /* extension: cap on the Newton iterations of one solve (one instance of a
+                            * batch), checked where max_iters is; 0 (default) = none.  A solve that
+                            * reaches it stops like one that reaches max_iters */
 } lassie_opts;
 
 typedef struct {
@@ -163,6 +166,12 @@ int lassie_batch_plan_info(const lassie_batch_plan *plan, int32_t *variant, int3
  * reference (it runs one model per process and reports its progress in the
  * ProgressSimulation file, Simulation.cuh:660-672). */
 int lassie_batch_plan_progress(const lassie_batch_plan *plan, int64_t *done);
+/* How the plan divides its instances between the batched form's two kernels
+ * and their device times in the last run (ms[0]: the wave-level kernel, one
+ * wavefront per instance of at most 64 species, FP64; ms[1]: the
+ * workgroup-level kernel with the copies out).  Any output may be NULL.  No
+ * counterpart in the reference (one model per process). */
+int lassie_batch_plan_timing(const lassie_batch_plan *plan, int32_t *n_wave, int32_t *n_wg, float *ms);
 int lassie_batch_plan_destroy(lassie_batch_plan *plan);
 
 /* Replaces the rate constants (c_vector) of an uploaded model, k[R].  A tape

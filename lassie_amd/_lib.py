@@ -46,7 +46,7 @@ class Opts(C.Structure):
                 ("stiffness_tol", C.c_double), ("rtol", C.c_double), ("newton_iter", C.c_int32),
                 ("bdf_regrow", C.c_int32), ("max_rkf_retries", C.c_int32),
                 ("precision", C.c_int32), ("max_iters", C.c_int64), ("refine_tol", C.c_double),
-                ("refine_max_iter", C.c_int32), ("reserved", C.c_int32)]
+                ("refine_max_iter", C.c_int32), ("reserved", C.c_int32), ("max_newton", C.c_int64)]
 
 
 class Stats(C.Structure):
@@ -70,7 +70,7 @@ EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy",
            "lassie_solve_batch", "lassie_model_set_rates", "lassie_forward_batch",
            "lassie_adjoint_batch", "lassie_tape_info", "lassie_tape_read", "lassie_tape_destroy",
            "lassie_batch_plan_create", "lassie_batch_plan_run", "lassie_batch_plan_info",
-           "lassie_batch_plan_progress", "lassie_batch_plan_destroy",
+           "lassie_batch_plan_progress", "lassie_batch_plan_timing", "lassie_batch_plan_destroy",
            "lassie_csr_from_dense", "lassie_device_count", "lassie_last_error", "lassie_abi_version")
 
 
@@ -114,6 +114,7 @@ def lib():
         L.lassie_batch_plan_run.argtypes = [C.c_void_p, _f64p, _f64p, C.POINTER(Stats), C.POINTER(C.c_double)]
         L.lassie_batch_plan_info.argtypes = [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_float)]
         L.lassie_batch_plan_progress.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.lassie_batch_plan_timing.argtypes = [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_float)]
         L.lassie_batch_plan_destroy.argtypes = [C.c_void_p]
         L.lassie_device_count.argtypes = [_i32p]
         L.lassie_last_error.restype = C.c_char_p
@@ -366,6 +367,16 @@ class BatchPlan:
         ms = (C.c_float * 3)()
         _check(lib().lassie_batch_plan_info(self.h, _p(v, _i32p), _p(rt, _i32p), ms))
         return int(v[0]), int(rt[0]), list(ms)
+
+    def timing(self):
+        """(instances on the wave-level kernel, instances on the workgroup-level
+        kernel, [ms of the wave-level kernel, ms of the workgroup-level kernel
+        with the copies out] in the last run)."""
+        nw = np.zeros(1, np.int32)
+        ng = np.zeros(1, np.int32)
+        ms = (C.c_float * 2)()
+        _check(lib().lassie_batch_plan_timing(self.h, _p(nw, _i32p), _p(ng, _i32p), ms))
+        return int(nw[0]), int(ng[0]), list(ms)
 
     def progress(self):
         """Instances finished by the run in flight (safe from another thread)."""

@@ -13,6 +13,7 @@
 #include "batch.h"
 #include "model.h"
 #include "solver.h"
+#include "wave.h"
 
 namespace {
 
@@ -144,6 +145,34 @@ void fill_batch_stats(const long long *c, const double *d, float ms, int slots, 
     s->kernel_ms = ms;
     s->groups = slots;
     for (int q = 0; q < 4; q++) s->phase_ms[q] = (double)c[14 + q] * 1e-5;
+}
+
+BatchSplit split_batch(lassie_model *const *models, const std::vector<BatchInst> &inst, bool allow_wave)
+{
+    BatchSplit sp;
+    const int n = (int)inst.size();
+    std::vector<int> wg, wv;
+    for (int i = 0; i < n; i++) {
+        const int nh = models[i]->hptr[models[i]->S];
+        if (allow_wave && inst[i].S <= kWaveMaxS && nh <= kWaveMaxNh) {
+            wv.push_back(i);
+            sp.smax_wave = std::max(sp.smax_wave, inst[i].S);
+            sp.nh_max_wave = std::max(sp.nh_max_wave, nh);
+        } else {
+            wg.push_back(i);
+            sp.smax_wg = std::max(sp.smax_wg, inst[i].S);
+        }
+    }
+    // most expensive first (dense LU work ~ S^3) so the tail of each queue is short
+    auto by_size = [&](int a, int b) { return inst[a].S > inst[b].S; };
+    std::stable_sort(wg.begin(), wg.end(), by_size);
+    std::stable_sort(wv.begin(), wv.end(), by_size);
+    sp.n_wg = (int)wg.size();
+    sp.n_wave = (int)wv.size();
+    sp.order = wg;
+    sp.order.insert(sp.order.end(), wv.begin(), wv.end());
+    if (sp.n_wave) sp.wave_lds = wave_lds_bytes(sp.smax_wave, sp.nh_max_wave);
+    return sp;
 }
 
 }  // namespace lassie
@@ -540,6 +569,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.bdf_regrow = o.bdf_regrow;
     a.max_rkf_retries = o.max_rkf_retries;
     a.max_iters = o.max_iters;
+    a.max_newton = o.max_newton;
     a.tv = m->d_tv;
     a.nt = nt;
     a.cs = m->d_cs;
@@ -689,15 +719,18 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
         y_tot += m->S;
         ckpt_tot += ((cap + lassie::kCkptEvery - 1) / lassie::kCkptEvery) * m->S;
     }
-    // most expensive first (dense LU work ~ S^3) so the tail of the queue is short
-    std::vector<int> order(n_models);
-    for (int i = 0; i < n_models; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].S > inst[b].S; });
+    const bool allow_wave = !tape && o.precision == LASSIE_PREC_FP64 && !(o.reserved & 2);
+    const lassie::BatchSplit sp = lassie::split_batch(models, inst, allow_wave);
+    const std::vector<int> &order = sp.order;
+    smax = sp.smax_wg;
 
-    int cus = 0, per_cu = 0;
+    int cus = 0, per_cu = 0, wave_per_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m0->device));
     if (lassie::batch_occupancy(&per_cu) != 0 || per_cu < 1) return fail(LASSIE_ERR_HIP, "batch occupancy query failed");
-    const int slots = std::max(1, std::min(n_models, cus * per_cu));
+    if (sp.n_wave && (lassie::wave_occupancy(sp.wave_lds, &wave_per_cu) != 0 || wave_per_cu < 1))
+        return fail(LASSIE_ERR_HIP, "wave kernel occupancy query failed");
+    const int slots = std::max(1, std::min(sp.n_wg, cus * per_cu));
+    const int wave_blocks = std::max(1, std::min(sp.n_wave, cus * wave_per_cu));
     const int Pmax = (smax + lassie::kBatchNB - 1) / lassie::kBatchNB;
     const size_t ws64_stride = 12 * (size_t)smax + (size_t)smax * smax + lassie::kLuSink;
     const size_t ws32_stride = (size_t)smax * smax + lassie::kLuSink;
@@ -717,7 +750,7 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     hipError_t e;
     if ((e = hipMalloc((void **)&d_inst, sizeof(lassie::BatchInst) * n_models)) != hipSuccess ||
         (e = hipMalloc((void **)&d_order, sizeof(int) * n_models)) != hipSuccess ||
-        (e = hipMalloc((void **)&d_next, sizeof(int))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_next, 2 * sizeof(int))) != hipSuccess ||
         (e = hipMalloc((void **)&d_wsi, sizeof(int) * wsi_stride * slots)) != hipSuccess ||
         (e = hipMalloc((void **)&d_tv, sizeof(double) * nt)) != hipSuccess ||
         (e = hipMalloc((void **)&d_out, sizeof(double) * out_tot)) != hipSuccess ||
@@ -746,12 +779,12 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     BATCH_TRY(hipMemcpyAsync(d_inst, inst.data(), sizeof(lassie::BatchInst) * n_models, hipMemcpyHostToDevice, st));
     BATCH_TRY(hipMemcpyAsync(d_order, order.data(), sizeof(int) * n_models, hipMemcpyHostToDevice, st));
     BATCH_TRY(hipMemcpyAsync(d_tv, t, sizeof(double) * nt, hipMemcpyHostToDevice, st));
-    BATCH_TRY(hipMemsetAsync(d_next, 0, sizeof(int), st));
+    BATCH_TRY(hipMemsetAsync(d_next, 0, 2 * sizeof(int), st));
     BATCH_TRY(hipMemsetAsync(d_wsi, 0, sizeof(int) * wsi_stride * slots, st));
     if (tape) BATCH_TRY(hipMemsetAsync(tape->d_sample, 0, sizeof(int) * (size_t)nt * n_models, st));
     lassie::BatchArgs a{};
     a.inst = d_inst;
-    a.n_inst = n_models;
+    a.n_inst = sp.n_wg;
     a.order = d_order;
     a.next = d_next;
     a.tv = d_tv;
@@ -769,6 +802,7 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     a.bdf_regrow = o.bdf_regrow;
     a.max_rkf_retries = o.max_rkf_retries;
     a.max_iters = o.max_iters;
+    a.max_newton = o.max_newton;
     a.precision = o.precision;
     a.refine_tol = o.refine_tol;
     a.refine_max_iter = std::max(1, o.refine_max_iter);
@@ -786,8 +820,17 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
         a.tape_ckpt = tape->d_ckpt;
         a.tape_cap = cap;
     }
+    // the wave-level kernel takes the small instances first, then the
+    // workgroup-level kernel the rest (two queues, same stream)
+    lassie::BatchArgs aw = a;
+    aw.n_inst = sp.n_wave;
+    aw.order = d_order + sp.n_wg;
+    aw.next = d_next + 1;
+    aw.s_max = sp.smax_wave;
+    aw.nh_max = sp.nh_max_wave;
     BATCH_TRY(hipEventRecord(m0->ev0, st));
-    BATCH_TRY(lassie::launch_batch(a, slots, st));
+    if (sp.n_wave) BATCH_TRY(lassie::launch_wave(aw, wave_blocks, sp.wave_lds, st));
+    if (sp.n_wg) BATCH_TRY(lassie::launch_batch(a, slots, st));
     BATCH_TRY(hipEventRecord(m0->ev1, st));
     BATCH_TRY(hipMemcpyAsync(out, d_out, sizeof(double) * out_tot, hipMemcpyDeviceToHost, st));
     if (y_final) BATCH_TRY(hipMemcpyAsync(y_final, d_y, sizeof(double) * y_tot, hipMemcpyDeviceToHost, st));

@@ -35,10 +35,12 @@ struct BatchArgs {
     double rkf_step, be_step, newton_tol, stiffness_tol, rtol;
     int newton_iter, bdf_regrow, max_rkf_retries;
     long long max_iters;
+    long long max_newton;  // 0: no cap on an instance's Newton iterations
     int precision;
     double refine_tol;
     int refine_max_iter;
     int s_max;
+    int nh_max;    // wave kernel: most net-stoichiometry entries of an instance (LDS sizing)
     double *ws64;  // per slot: 12 vectors + dense M (FP64)
     size_t ws64_stride;
     float *ws32;   // per slot: dense M (FP32)
@@ -66,6 +68,7 @@ constexpr int kBatchDefaultVariant = 2;
 int batch_variant_rt(int variant);
 hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream, int variant = kBatchDefaultVariant);
 int batch_occupancy(int *per_cu, int variant = kBatchDefaultVariant);
+// Resets both work-queue heads (next[0], next[1]) inside a captured graph.
 hipError_t launch_batch_reset(int *next, hipStream_t stream);
 
 }  // namespace lassie

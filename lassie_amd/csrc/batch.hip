@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
         };
 
         while (true) {
-            if (A.max_iters > 0 && st_loop >= A.max_iters) {
+            if ((A.max_iters > 0 && st_loop >= A.max_iters) || (A.max_newton > 0 && st_newton >= A.max_newton)) {
                 rc = 1;
                 break;
             }
@@ -402,10 +402,10 @@ hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream, in
     return hipGetLastError();
 }
 
-// Resets the work-queue head inside a captured graph (a kernel node, not a memset node).
+// Resets the work-queue heads inside a captured graph (a kernel node, not a memset node).
 __global__ void batch_reset_kernel(int *next)
 {
-    if (threadIdx.x == 0) *next = 0;
+    if (threadIdx.x < 2) next[threadIdx.x] = 0;
 }
 
 hipError_t launch_batch_reset(int *next, hipStream_t stream)

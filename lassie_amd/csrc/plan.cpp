@@ -22,6 +22,7 @@
 #include "../../include/lassie.h"
 #include "batch.h"
 #include "model.h"
+#include "wave.h"
 
 namespace {
 
@@ -32,6 +33,8 @@ std::map<std::tuple<int, int, int, int>, int> g_tuned;  // (device, largest S, c
 
 struct lassie_batch_plan {
     int device = 0, n = 0, nt = 0, smax = 1, slots = 1, variant = lassie::kBatchDefaultVariant;
+    int n_wg = 0, n_wave = 0, wave_blocks = 0;
+    size_t wave_lds = 0;
     lassie_opts opts{};
     long long out_tot = 0, y_tot = 0;
     lassie::BatchInst *d_inst = nullptr;
@@ -43,17 +46,21 @@ struct lassie_batch_plan {
     long long *h_st = nullptr;
     unsigned int *h_done = nullptr;  // per slot: instances finished in the current run
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
+    // graph 0: queue reset + wave-level kernel; graph 1: workgroup-level kernel + copies out
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     float tune_ms[lassie::kBatchVariants] = {0};
-    lassie::BatchArgs args{};
+    float last_ms[2] = {0.f, 0.f};  // device time of the last run's two kernels
+    lassie::BatchArgs args{}, wargs{};
 };
 
 static void free_plan(lassie_batch_plan *p)
 {
-    if (p->exec) (void)hipGraphExecDestroy(p->exec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (int g = 0; g < 2; g++) {
+        if (p->exec[g]) (void)hipGraphExecDestroy(p->exec[g]);
+        if (p->graph[g]) (void)hipGraphDestroy(p->graph[g]);
+    }
     void *dptrs[] = {p->d_inst, p->d_order, p->d_next, p->d_wsi, p->d_tv, p->d_out,
                      p->d_y,    p->d_ws64,  p->d_dst,  p->d_ws32, p->d_st};
     for (void *q : dptrs)
@@ -62,6 +69,7 @@ static void free_plan(lassie_batch_plan *p)
     for (void *q : hptrs)
         if (q) (void)hipHostFree(q);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->evm) (void)hipEventDestroy(p->evm);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -105,25 +113,33 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     P->n = n_models;
     P->nt = nt;
     if (hipSetDevice(P->device) != hipSuccess) return bail(LASSIE_ERR_HIP, "hipSetDevice failed");
-    std::vector<int> order(n_models);
-    for (int i = 0; i < n_models; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].S > inst[b].S; });
-    int cus = 0, per_cu = 0;
+    const lassie::BatchSplit sp =
+        lassie::split_batch(models, inst, o.precision == LASSIE_PREC_FP64 && !(o.reserved & 2));
+    const std::vector<int> &order = sp.order;
+    P->n_wg = sp.n_wg;
+    P->n_wave = sp.n_wave;
+    P->smax = sp.smax_wg;
+    P->wave_lds = sp.wave_lds;
+    int cus = 0, per_cu = 0, wave_per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device) != hipSuccess)
         return bail(LASSIE_ERR_HIP, "device attribute query failed");
     if (lassie::batch_occupancy(&per_cu) != 0 || per_cu < 1) return bail(LASSIE_ERR_HIP, "batch occupancy query failed");
-    P->slots = std::max(1, std::min(n_models, cus * per_cu));
+    if (sp.n_wave && (lassie::wave_occupancy(sp.wave_lds, &wave_per_cu) != 0 || wave_per_cu < 1))
+        return bail(LASSIE_ERR_HIP, "wave kernel occupancy query failed");
+    P->slots = std::max(1, std::min(sp.n_wg, cus * per_cu));
+    P->wave_blocks = std::max(1, std::min(sp.n_wave, cus * wave_per_cu));
     const int smax = P->smax, slots = P->slots;
+    const int n_done = slots + P->wave_blocks;  // finished-instance counters: workgroup slots, then waves
     const int Pmax = (smax + lassie::kBatchNB - 1) / lassie::kBatchNB;
     const size_t ws64_stride = 12 * (size_t)smax + (size_t)smax * smax + lassie::kLuSink;
     const size_t ws32_stride = (size_t)smax * smax + lassie::kLuSink;
     const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
     hipError_t e;
     if ((e = hipStreamCreate(&P->stream)) != hipSuccess || (e = hipEventCreate(&P->ev0)) != hipSuccess ||
-        (e = hipEventCreate(&P->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&P->evm)) != hipSuccess || (e = hipEventCreate(&P->ev1)) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_inst, sizeof(lassie::BatchInst) * n_models)) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_order, sizeof(int) * n_models)) != hipSuccess ||
-        (e = hipMalloc((void **)&P->d_next, sizeof(int))) != hipSuccess ||
+        (e = hipMalloc((void **)&P->d_next, 2 * sizeof(int))) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_wsi, sizeof(int) * wsi_stride * slots)) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_tv, sizeof(double) * nt)) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_out, sizeof(double) * P->out_tot)) != hipSuccess ||
@@ -138,7 +154,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         (e = hipHostMalloc((void **)&P->h_dst, sizeof(double) * 4 * n_models)) != hipSuccess ||
         (e = hipHostMalloc((void **)&P->h_st, sizeof(long long) * lassie::kBatchStats * n_models)) != hipSuccess ||
         // coherent: the kernel's system-scope stores reach the host while it runs
-        (e = hipHostMalloc((void **)&P->h_done, sizeof(unsigned int) * slots, hipHostMallocCoherent)) !=
+        (e = hipHostMalloc((void **)&P->h_done, sizeof(unsigned int) * n_done, hipHostMallocCoherent)) !=
             hipSuccess)
         return bail(LASSIE_ERR_NOMEM, std::string("batch plan: ") + hipGetErrorString(e));
     if ((e = hipMemcpy(P->d_inst, inst.data(), sizeof(lassie::BatchInst) * n_models, hipMemcpyHostToDevice)) !=
@@ -149,7 +165,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         return bail(LASSIE_ERR_HIP, std::string("batch plan upload: ") + hipGetErrorString(e));
     lassie::BatchArgs &a = P->args;
     a.inst = P->d_inst;
-    a.n_inst = n_models;
+    a.n_inst = sp.n_wg;
     a.order = P->d_order;
     a.next = P->d_next;
     a.tv = P->d_tv;
@@ -167,6 +183,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     a.bdf_regrow = o.bdf_regrow;
     a.max_rkf_retries = o.max_rkf_retries;
     a.max_iters = o.max_iters;
+    a.max_newton = o.max_newton;
     a.precision = o.precision;
     a.refine_tol = o.refine_tol;
     a.refine_max_iter = std::max(1, o.refine_max_iter);
@@ -177,7 +194,15 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     a.ws32_stride = ws32_stride;
     a.wsi = P->d_wsi;
     a.wsi_stride = wsi_stride;
-    std::memset(P->h_done, 0, sizeof(unsigned int) * slots);
+    std::memset(P->h_done, 0, sizeof(unsigned int) * n_done);
+    lassie::BatchArgs &aw = P->wargs;
+    aw = a;
+    aw.n_inst = sp.n_wave;
+    aw.order = P->d_order + sp.n_wg;
+    aw.next = P->d_next + 1;
+    aw.s_max = sp.smax_wave;
+    aw.nh_max = sp.nh_max_wave;
+    aw.done_host = P->h_done + slots;
 
     // autotuner: a short trial (64 loop iterations per instance) of every
     // variant; the fastest is kept for this (device, largest S, count, precision)
@@ -188,6 +213,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) chosen = it->second;
     }
+    if (chosen < 0 && sp.n_wg == 0) chosen = lassie::kBatchDefaultVariant;
     if (chosen < 0) {
         lassie::BatchArgs trial = a;
         trial.max_iters = 64;
@@ -214,25 +240,36 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     P->variant = chosen;
     a.done_host = P->h_done;
 
-    // the launch sequence as a graph (no memset nodes: the queue head is reset by a kernel)
-    if ((e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-        return bail(LASSIE_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
-    hipError_t ec = lassie::launch_batch_reset(P->d_next, P->stream);
-    if (ec == hipSuccess) ec = lassie::launch_batch(a, slots, P->stream, P->variant);
-    if (ec == hipSuccess)
-        ec = hipMemcpyAsync(P->h_out, P->d_out, sizeof(double) * P->out_tot, hipMemcpyDeviceToHost, P->stream);
-    if (ec == hipSuccess)
-        ec = hipMemcpyAsync(P->h_y, P->d_y, sizeof(double) * P->y_tot, hipMemcpyDeviceToHost, P->stream);
-    if (ec == hipSuccess)
-        ec = hipMemcpyAsync(P->h_st, P->d_st, sizeof(long long) * lassie::kBatchStats * n_models,
-                            hipMemcpyDeviceToHost, P->stream);
-    if (ec == hipSuccess)
-        ec = hipMemcpyAsync(P->h_dst, P->d_dst, sizeof(double) * 4 * n_models, hipMemcpyDeviceToHost, P->stream);
-    e = hipStreamEndCapture(P->stream, &P->graph);
-    if (ec != hipSuccess || e != hipSuccess)
-        return bail(LASSIE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec != hipSuccess ? ec : e));
-    if ((e = hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0)) != hipSuccess)
-        return bail(LASSIE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    // the launch sequence as two graphs (no memset nodes: the queue heads are
+    // reset by a kernel): [reset, wave-level kernel] and [workgroup-level
+    // kernel, copies out]; an event between them times each kernel
+    for (int g = 0; g < 2; g++) {
+        if ((e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return bail(LASSIE_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+        hipError_t ec = hipSuccess;
+        if (g == 0) {
+            ec = lassie::launch_batch_reset(P->d_next, P->stream);
+            if (ec == hipSuccess && sp.n_wave) ec = lassie::launch_wave(aw, P->wave_blocks, sp.wave_lds, P->stream);
+        } else {
+            if (sp.n_wg) ec = lassie::launch_batch(a, slots, P->stream, P->variant);
+            if (ec == hipSuccess)
+                ec = hipMemcpyAsync(P->h_out, P->d_out, sizeof(double) * P->out_tot, hipMemcpyDeviceToHost, P->stream);
+            if (ec == hipSuccess)
+                ec = hipMemcpyAsync(P->h_y, P->d_y, sizeof(double) * P->y_tot, hipMemcpyDeviceToHost, P->stream);
+            if (ec == hipSuccess)
+                ec = hipMemcpyAsync(P->h_st, P->d_st, sizeof(long long) * lassie::kBatchStats * n_models,
+                                    hipMemcpyDeviceToHost, P->stream);
+            if (ec == hipSuccess)
+                ec = hipMemcpyAsync(P->h_dst, P->d_dst, sizeof(double) * 4 * n_models, hipMemcpyDeviceToHost,
+                                    P->stream);
+        }
+        e = hipStreamEndCapture(P->stream, &P->graph[g]);
+        if (ec != hipSuccess || e != hipSuccess)
+            return bail(LASSIE_ERR_HIP,
+                        std::string("graph capture: ") + hipGetErrorString(ec != hipSuccess ? ec : e));
+        if ((e = hipGraphInstantiate(&P->exec[g], P->graph[g], nullptr, nullptr, 0)) != hipSuccess)
+            return bail(LASSIE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
     *out = P;
     return LASSIE_OK;
 }
@@ -244,13 +281,17 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
     if (!P || !out) return api_fail(LASSIE_ERR_INVALID, "plan and out are required");
     hipError_t e;
     // no launch of this plan is in flight here (runs are synchronous)
-    std::memset(P->h_done, 0, sizeof(unsigned int) * P->slots);
+    std::memset(P->h_done, 0, sizeof(unsigned int) * (P->slots + P->wave_blocks));
     if ((e = hipSetDevice(P->device)) != hipSuccess || (e = hipEventRecord(P->ev0, P->stream)) != hipSuccess ||
-        (e = hipGraphLaunch(P->exec, P->stream)) != hipSuccess || (e = hipEventRecord(P->ev1, P->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(P->stream)) != hipSuccess)
+        (e = hipGraphLaunch(P->exec[0], P->stream)) != hipSuccess ||
+        (e = hipEventRecord(P->evm, P->stream)) != hipSuccess ||
+        (e = hipGraphLaunch(P->exec[1], P->stream)) != hipSuccess ||
+        (e = hipEventRecord(P->ev1, P->stream)) != hipSuccess || (e = hipStreamSynchronize(P->stream)) != hipSuccess)
         return api_fail(LASSIE_ERR_HIP, std::string("plan run: ") + hipGetErrorString(e));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, P->ev0, P->ev1);
+    (void)hipEventElapsedTime(&P->last_ms[0], P->ev0, P->evm);
+    (void)hipEventElapsedTime(&P->last_ms[1], P->evm, P->ev1);
     if (elapsed_ms) *elapsed_ms = ms;
     std::memcpy(out, P->h_out, sizeof(double) * P->out_tot);
     if (y_final) std::memcpy(y_final, P->h_y, sizeof(double) * P->y_tot);
@@ -260,7 +301,21 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
         if (stats) lassie::fill_batch_stats(c, P->h_dst + 4 * (size_t)i, ms, P->slots, &stats[i]);
         maxit |= c[9] == 1;
     }
-    if (maxit) return api_fail(LASSIE_ERR_MAXITER, "max_iters reached before the last sample in at least one instance");
+    if (maxit)
+        return api_fail(LASSIE_ERR_MAXITER,
+                        "max_iters or max_newton reached before the last sample in at least one instance");
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_batch_plan_timing(const lassie_batch_plan *P, int32_t *n_wave, int32_t *n_wg, float *ms)
+{
+    if (!P) return lassie::api_fail(LASSIE_ERR_INVALID, "plan is NULL");
+    if (n_wave) *n_wave = P->n_wave;
+    if (n_wg) *n_wg = P->n_wg;
+    if (ms) {
+        ms[0] = P->last_ms[0];
+        ms[1] = P->last_ms[1];
+    }
     return LASSIE_OK;
 }
 
@@ -269,7 +324,7 @@ extern "C" int lassie_batch_plan_progress(const lassie_batch_plan *P, int64_t *d
     using lassie::api_fail;
     if (!P || !done) return api_fail(LASSIE_ERR_INVALID, "plan and done are required");
     int64_t sum = 0;
-    for (int s = 0; s < P->slots; s++)
+    for (int s = 0; s < P->slots + P->wave_blocks; s++)
         sum += __atomic_load_n(P->h_done + s, __ATOMIC_RELAXED);
     *done = sum;
     return LASSIE_OK;

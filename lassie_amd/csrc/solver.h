@@ -40,6 +40,7 @@ struct SolveArgs {
     double rkf_step, be_step, newton_tol, stiffness_tol, rtol;
     int newton_iter, bdf_regrow, max_rkf_retries;
     long long max_iters;
+    long long max_newton;  // 0: no cap on the solve's Newton iterations
     const double *tv;
     int nt;
     const int *cs;

@@ -854,7 +854,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
             tmark = wall_clock64();
             int mode = MODE_CONTINUE;
             T hb = 0.0;
-            if (A.max_iters > 0 && st_loop >= A.max_iters) {
+            if ((A.max_iters > 0 && st_loop >= A.max_iters) || (A.max_newton > 0 && st_newton >= A.max_newton)) {
                 mode = MODE_DONE;
                 rc = 1;
             } else {

@@ -48,6 +48,8 @@
  *     shrinks it, Simulation.cuh:872, 907);
  *   max_rkf_retries > 0: that many consecutive rejected RKF attempts send the
  *     next rejection to BDF (the reference can cycle on rejections forever);
+ *   max_newton > 0: the loop stops (as at max_iters) once the solve has run
+ *     that many modified-Newton iterations, a work bound per solve;
  *   refine: the precision form of the Newton linear solve (orc_refine_solve):
  *     FP32 factors of (float)(I - hJ), FP64 iterative refinement, FP64 LU of
  *     the step's matrix when the FP32 LU meets a zero pivot or refinement
@@ -517,6 +519,7 @@ int orc_solve_rec(const orc_model *m, const orc_opts *o, const double *tv, int n
     if (tape) { tape->n = 0; tape->overflow = 0; }
     while (i < nt) {
         if (o->max_iters > 0 && st->loop_iters >= o->max_iters) { rc = 1; break; }
+        if (o->max_newton > 0 && st->newton_iters >= o->max_newton) { rc = 1; break; }
         if (progress && st->loop_iters < progress_cap) progress[st->loop_iters] = t / t_max;
         st->loop_iters++;
         if (fabs(t1 - t) < 1e-5) {

@@ -32,6 +32,8 @@ typedef struct {
     double refine_tol;    /* precision form: refinement stop, |r_i| <= tol*(|rhs_i| + (|M||x|)_i + atol_i) */
     int32_t refine;       /* precision form: 1 = FP32 LU + FP64 refinement, 0 = FP64 LU */
     int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback */
+    int64_t max_newton;   /* extension: cap on the solve's Newton iterations, checked with
+                           * max_iters at the top of the loop; 0 = none */
 } orc_opts;
 
 typedef struct {

@@ -386,6 +386,7 @@ int orc32_solve(const orc_model *m, const orc_opts *o, const double *tv, int nt,
     int64_t nrej = 0;
     while (i < nt) {
         if (o->max_iters > 0 && st->loop_iters >= o->max_iters) { rc = 1; break; }
+        if (o->max_newton > 0 && st->newton_iters >= o->max_newton) { rc = 1; break; }
         if (progress && st->loop_iters < progress_cap) progress[st->loop_iters] = t / t_max;
         st->loop_iters++;
         if (fabsf(t1 - t) < 1e-5) {

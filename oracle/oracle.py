@@ -31,7 +31,7 @@ class OrcOpts(C.Structure):
                 ("stiffness_tol", C.c_double), ("rtol", C.c_double), ("newton_iter", C.c_int32),
                 ("bdf_regrow", C.c_int32), ("max_rkf_retries", C.c_int32), ("pad_", C.c_int32),
                 ("max_iters", C.c_int64), ("refine_tol", C.c_double), ("refine", C.c_int32),
-                ("refine_max_iter", C.c_int32)]
+                ("refine_max_iter", C.c_int32), ("max_newton", C.c_int64)]
 
 
 class OrcStats(C.Structure):
@@ -144,11 +144,11 @@ class _Bound:
 
 def opts(rkf_step=1e-3, be_step=1e-2, newton_tol=1e-6, stiffness_tol=1e-6, rtol=0.0,
          newton_iter=10000, bdf_regrow=0, max_rkf_retries=0, max_iters=0, refine=0,
-         refine_tol=1e-14, refine_max_iter=4):
+         refine_tol=1e-14, refine_max_iter=4, max_newton=0):
     """Controller options; refine=1 selects the precision form (FP32 LU of
     (float)M with FP64 refinement, orc_refine_solve)."""
     return OrcOpts(rkf_step, be_step, newton_tol, stiffness_tol, rtol, newton_iter, bdf_regrow,
-                   max_rkf_retries, 0, max_iters, refine_tol, refine, refine_max_iter)
+                   max_rkf_retries, 0, max_iters, refine_tol, refine, refine_max_iter, max_newton)
 
 
 def rhs(m, x):

@@ -20,7 +20,7 @@ def test_every_declared_symbol_is_exported():
     lib = C.CDLL(_lib.LIB_PATH)
     for name in declared:
         assert hasattr(lib, name), name
-    assert _lib.lib().lassie_abi_version() == 1
+    assert _lib.lib().lassie_abi_version() == 2
 
 
 def test_option_defaults_are_the_references():
@@ -29,7 +29,7 @@ def test_option_defaults_are_the_references():
     assert (o.rkf_step, o.be_step, o.newton_iter, o.newton_tol, o.stiffness_tol) == \
         (1e-3, 1e-2, 10000, 1e-6, 1e-6)
     assert o.rtol == 0.0 and o.bdf_regrow == 0 and o.max_rkf_retries == 0
-    assert o.precision == _lib.PREC_FP64
+    assert o.precision == _lib.PREC_FP64 and o.max_newton == 0
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
