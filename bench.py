@@ -1,32 +1,41 @@
-"""Throughput benchmark: ODE solves/sec on one MI355X, LASSIE's plain FP64
-method (RKF45 / BDF1-Newton loop), plus the chosen (batched) form.
+"""Throughput benchmark: ODE solves/sec on one MI355X, the chosen form (the
+batched FP64 plan) against LASSIE's plain FP64 method on the same models.
 
-Headline workload (one "step" = one full solve, every sample written):
-  1024 species x 1024 reactions from the seeded generator (seed 0), k
-  log-uniform in [1e-6, 1e6], x0 log-uniform in [1e-6, 1], t in [0, 100] with
-  200 samples, rtol 1e-6 (tol = atol + rtol*|w|), atol 1e-12, BDF step
-  regrowth on, at most 8 RKF rejections in a row.  The model is uploaded once,
-  outside the timed region; the solve runs as one persistent kernel.
+Workload (one "step" = one pass over the whole batch, every sample written):
+  2048 independent instances of the uneven stiff generator, S drawn uniformly
+  from {16, 32, 64, 128, 256, 512}, R = 2S, instance i from seed 1000 + i,
+  k log-uniform in [1e-4, 1e4], x0 log-uniform in [1e-6, 1]; t in [0, 50] with
+  100 samples; controller rtol 1e-6 (tol = atol + rtol*|w|), atol 1e-12, BDF
+  step regrowth, at most 8 RKF rejections in a row, and at most 50000
+  modified-Newton iterations per instance (the same cap in every form: about
+  1% of the instances reach it, see `capped`).  Models are uploaded and the
+  plan prepared outside the timed region; a step replays the plan (the
+  workgroup-level kernel for S > 64 and, concurrently, the wave-level kernel
+  for S <= 64) and copies every trajectory back.
 
-Chosen form (reported beside the headline, not as `value`): the batched
-persistent kernel, prepared as a plan (graph replay, autotuned kernel variant),
-on uneven instances (S in {16..512}, R = 2S, k logU[1e-4, 1e4], t in [0, 50],
-100 samples), against the plain single-model path on a subsample of the same
-instances (their trajectories must agree bit for bit).
+Plain form: the single-model persistent solver (one launch per instance), run
+on a stratified sample of the same instances (the first instances of every
+size); its trajectories must equal the chosen form's bit for bit, and its
+workload rate is projected from the per-size mean times.
 
-Differentiation form (also beside the headline): forward pass with step record
-plus discrete-adjoint gradient of a log-loss, for B parameterisations of one
-512 x 1024 network on a short horizon.
+Roofline: the workgroup-level kernel (the dominant one): algorithmic FP64
+work (2S^3/3 per LU, 2S^2 per Newton solve) of its instances over its
+device time from HIP events on its stream, against the FP64 peak; `traffic`
+is its measured HBM bytes per launch (profiles/traffic_batch_c4.json).
 
-CPU baselines: the LSODA path (scipy odeint, the reference's CPU method) on the
-same 1024x1024 model, one thread, full solve; and the oracle (the same
-algorithm on the CPU, one thread) projected from its measured per-operation
-cost and the GPU run's operation counts.
+CPU baselines (rank 0 only, before the GPU is touched, one core each):
+  cpu_baseline   the oracle (the same FP64 RKF45/BDF1 algorithm on the CPU) on a
+                 bounded stratified sample: full solves for S <= 128, a prefix
+                 of the loop for S = 256 and 512 scaled by each instance's full
+                 BDF-step count from the GPU run;
+  cpu_lsoda      LSODA (scipy odeint, restating the reference's CPU-based
+                 LSODA/LSODAsim.py) on the same sample, a different algorithm.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -34,332 +43,286 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ODE solves/sec at rtol 1e-6, chosen form vs plain FP64 on MI355X; max rel err"
 FP64_PEAK_TFLOPS = 78.6
-WORKLOAD = dict(S=1024, R=1024, seed=0, k_range=(1e-6, 1e6), t_end=100.0, n_samples=200)
-CONTROLLER = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
-
-
-def flops_per_solve(n, st):
-    """Algorithmic FP64 work of one solve: one LU per BDF step (2n^3/3) and one
-    pair of triangular solves per Newton iteration (2n^2)."""
-    return st["lu_factorizations"] * (2.0 * n ** 3 / 3.0) + st["newton_iters"] * 2.0 * n * n
-
-
-def cpu_lsoda(m, t, out_gpu):
-    import numpy as np
-    from oracle import lsoda
-    t0 = time.perf_counter()
-    y = lsoda.solve(m, t, rtol=1e-6)
-    dt = time.perf_counter() - t0
-    rel = np.abs(out_gpu - y) / (np.abs(y) + 1e-12)
-    sig = np.abs(y) > 1e-6 * np.abs(y).max()
-    return dt, float(rel.max()), float(np.median(rel[sig]))
-
-
-def cpu_same_algorithm(m, st):
-    """Per-operation cost of the oracle on this model (one LU + solve at the
-    initial state, one RKF45 attempt), scaled by the GPU run's counts."""
-    import numpy as np
-    from oracle import oracle as O
-    x = np.asarray(m.x0, float)
-    t0 = time.perf_counter()
-    O.rkf_attempt(m, x, 1e-3, rtol=1e-6)
-    t_rkf = time.perf_counter() - t0
-    J = O.jac(m, x)
-    t0 = time.perf_counter()
-    M = O.newton_matrix(O.jac(m, x), 0.01)
-    LU, piv, _ = O.lu(M)
-    t_lu = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for _ in range(3):
-        O.lu_solve(LU, piv, x)
-        O.rhs(m, x)
-    t_sol = (time.perf_counter() - t0) / 3
-    del J
-    proj = st["bdf_steps"] * t_lu + st["newton_iters"] * t_sol + st["rkf_attempts"] * t_rkf
-    return proj, dict(lu_s=t_lu, newton_iter_s=t_sol, rkf_attempt_s=t_rkf)
-
-
-def cpu_batched(models, t, st, sample):
-    """CPU baselines of the batched form, on one core (this process has
-    initialised the GPU, so it starts no worker processes): LSODA (the
-    reference's CPU method, scipy odeint) on the first `sample` instances, and
-    the oracle's per-operation cost for each size times the GPU run's
-    per-instance operation counts (the same algorithm)."""
-    import numpy as np
-    from oracle import lsoda
-    from oracle import oracle as O
-    t0 = time.perf_counter()
-    for m in models[:sample]:
-        lsoda.solve(m, t, rtol=1e-6)
-    wall = time.perf_counter() - t0
-    unit = {}
-    for m in models:
-        if m.S in unit:
-            continue
-        x = np.asarray(m.x0, float)
-        t0 = time.perf_counter()
-        O.rkf_attempt(m, x, 1e-3, rtol=1e-6)
-        t_rkf = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        LU, piv, _ = O.lu(O.newton_matrix(O.jac(m, x), 0.01))
-        t_lu = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        for _ in range(3):
-            O.lu_solve(LU, piv, x)
-            O.rhs(m, x)
-        unit[m.S] = (t_lu, (time.perf_counter() - t0) / 3, t_rkf)
-    proj = sum(d["bdf_steps"] * unit[m.S][0] + d["newton_iters"] * unit[m.S][1]
-               + d["rkf_attempts"] * unit[m.S][2] for m, d in zip(models, st))
-    return {"lsoda": {"value": sample / wall, "unit": "solves/s", "cores": 1, "kind": "port",
-                      "sample": f"the first {sample} of the {len(models)} instances, LSODA (scipy odeint "
-                                f"restating CPU-based LSODA/LSODA.py; rtol 1e-6, atol 1e-12), one core; "
-                                f"independent instances, so N cores give N times this",
-                      "seconds": wall},
-            "same_algorithm_projection": {
-                "value": len(models) / proj, "unit": "solves/s", "cores": 1, "kind": "port",
-                "sample": "oracle per-operation cost per size (one LU, one Newton solve + RHS, one "
-                          "RKF45 attempt) x the GPU run's per-instance counts", "projected_seconds": proj}}
-
-
-def chosen_form(n_batch, n_plain, with_plain, with_cpu=False):
-    """Batched FP64 form (a plan: graph replay, autotuned kernel variant) vs
-    the plain path on the same instances."""
-    from lassie_amd import _lib, synth
-    t0 = time.perf_counter()
-    models = synth.uneven_batch(n_batch, 1000)
-    t = synth.t_vector(0.0, 50.0, 100)
-    say(f"  {n_batch} models generated ({time.perf_counter() - t0:.1f} s)")
-    dms = []
-    for i, m in enumerate(models):
-        dms.append(_lib.DeviceModel(m))
-        if (i + 1) % 256 == 0:
-            say(f"  {i + 1} models uploaded ({time.perf_counter() - t0:.1f} s)")
-    plan = _lib.BatchPlan(dms, t, **CONTROLLER)
-    variant, tile_rows, trial_ms = plan.info()
-    say(f"  plan prepared, variant {variant} ({time.perf_counter() - t0:.1f} s)")
-    # the run blocks this thread; another one reports each time the device's
-    # count of finished instances has grown by 1/64 of the batch
-    import threading
-    finished = threading.Event()
-
-    def follow(t_start):
-        step, last = max(1, n_batch // 64), 0
-        while not finished.wait(2.0):
-            done = plan.progress()
-            if done >= last + step:
-                last = done
-                say(f"  {done} of {n_batch} instances finished ({time.perf_counter() - t_start:.1f} s)")
-
-    t0 = time.perf_counter()
-    watcher = threading.Thread(target=follow, args=(t0,), daemon=True)
-    watcher.start()
-    try:
-        outs, _, st, ms, rc = plan.run()
-    finally:
-        finished.set()
-        watcher.join()
-    wall = time.perf_counter() - t0
-    say(f"  batched run: {wall:.2f} s")
-    plan.close()
-    res = {"instances": n_batch, "sizes": sorted({m.S for m in models}),
-           "batched_solves_per_s": n_batch / wall, "batched_device_ms": ms,
-           "precision": "f64 (bit-identical to the oracle)",
-           "autotuner": {"variant": variant, "lu_tile_rows_in_flight": tile_rows,
-                         "trial_ms_per_variant": trial_ms},
-           "launch": "hipGraph replay of a prepared plan",
-           "note": "C4 names 2048 instances; --batch 2048 measures them, but its makespan is set by a "
-                   "few instances that the LASSIE controller drives into tens of millions of "
-                   "modified-Newton iterations, so the default measures the first 256"}
-    if with_plain:
-        plain_s = 0.0
-        for dm, ob in zip(dms[:n_plain], outs[:n_plain]):
-            t0 = time.perf_counter()
-            out, _, _, _, _ = dm.solve(t, **CONTROLLER)
-            plain_s += time.perf_counter() - t0
-            assert (out == ob).all(), "batched and plain forms disagree"
-        res["plain_subsample"] = n_plain
-        res["plain_solves_per_s"] = n_plain / plain_s
-        res["speedup_vs_plain"] = res["batched_solves_per_s"] / res["plain_solves_per_s"]
-        res["max_rel_err_vs_plain"] = 0.0
-    for dm in dms:
-        dm.close()
-    if with_cpu:
-        say("  CPU baselines of the batched form (one core)")
-        res["cpu_baseline"] = cpu_batched(models, t, st, min(32, n_batch))
-    return res
-
-
-def differentiation_form(B, t_end):
-    """Forward + discrete-adjoint gradient of the log-loss for B
-    parameterisations of one 512 x 1024 network (k logU[1e-4, 1e4], 50
-    samples on [0, t_end]); reports gradients/s."""
-    import numpy as np
-    from lassie_amd import _lib, synth
-    models, hidden = synth.parameterisations(512, 1024, B, seed=0)
-    t = synth.t_vector(0.0, t_end, 50)
-    dms = [_lib.DeviceModel(m) for m in models]
-    eps = 1e-12
-    t0 = time.perf_counter()
-    outs, st, fwd_ms, tape = _lib.forward_batch(dms, t, max_steps=1 << 16, **CONTROLLER)
-    target = [o * np.exp(0.1) for o in outs]  # fixed synthetic target
-    gouts = [2.0 * (np.log(o + eps) - np.log(tg + eps)) / (o + eps) for o, tg in zip(outs, target)]
-    gks, _, adj_ms = tape.adjoint(gouts)
-    wall = time.perf_counter() - t0
-    steps = [tape.info(i)[0] for i in range(B)]
-    tape.close()
-    for dm in dms:
-        dm.close()
-    return {"parameterisations": B, "network": "512 species x 1024 reactions", "t": [0.0, t_end],
-            "samples": 50, "gradients_per_s": B / wall, "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
-            "mean_steps": float(np.mean(steps)), "checkpoint_every": 32,
-            "finite_fraction": float(np.mean([np.isfinite(g).all() for g in gks]))}
-
-
-def precision_form_4096(cap):
-    """The 4096 x 4096 configuration (same generator as the headline, seed 0)
-    in the plain FP64 form and in the single-model precision form (FP32 grid
-    LU, FP64 refinement to a componentwise 1e-14 with the species' atol as
-    floor), over the first `cap` loop iterations of the same controller."""
-    import numpy as np
-    from lassie_amd import _lib, synth
-    m = synth.stiff(4096, 4096, 0, WORKLOAD["k_range"])
-    t = synth.t_vector(0.0, WORKLOAD["t_end"], WORKLOAD["n_samples"])
-    dm = _lib.DeviceModel(m)
-    res, finals = {}, {}
-    for name, kw in (("fp64", {}), ("fp32_refine", dict(precision=_lib.PREC_FP32_REFINE))):
-        say(f"  4096x4096 {name}, first {cap} loop iterations")
-        t0 = time.perf_counter()
-        _, yf, st, rc, _ = dm.solve(t, raise_on_maxiter=False, max_iters=cap, **CONTROLLER, **kw)
-        wall = time.perf_counter() - t0
-        finals[name] = yf
-        lu = max(st["lu_factorizations"], 1)
-        res[name] = {"wall_s": wall, "kernel_ms": st["kernel_ms"], "bdf_steps": st["bdf_steps"],
-                     "newton_iters": st["newton_iters"], "lu_factorizations": st["lu_factorizations"],
-                     "lu_ms_per_factorization": st["phase_ms"][2] / lu,
-                     "newton_ms_per_iteration": st["phase_ms"][3] / max(st["newton_iters"], 1),
-                     "refine_sweeps": st["refine_iters"], "refine_fallbacks": st["refine_fallbacks"],
-                     "groups": st["groups"]}
-        say(f"  {name}: {wall:.1f} s")
-    dm.close()
-    a, b = finals["fp64"], finals["fp32_refine"]
-    res["max_rel_err_final_state"] = float(np.max(np.abs(b - a) / (np.abs(a) + m.atol)))
-    res["loop_iterations"] = cap
-    return res
+WORKLOAD = dict(n=2048, seed_base=1000, sizes=(16, 32, 64, 128, 256, 512), k_range=(1e-4, 1e4),
+                t_end=50.0, n_samples=100)
+CONTROLLER = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8, max_newton=50000)
+T_START = time.perf_counter()
 
 
 def say(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T_START:6.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
+def elapsed():
+    return time.perf_counter() - T_START
+
+
+def work_flops(S, st):
+    """Algorithmic FP64 work of one instance: one LU per factorization
+    (2S^3/3) and one pair of triangular solves per Newton iteration (2S^2)."""
+    return st["lu_factorizations"] * (2.0 * S ** 3 / 3.0) + st["newton_iters"] * 2.0 * S * S
+
+
+def workload():
+    from lassie_amd import synth
+    w = WORKLOAD
+    models = synth.uneven_batch(w["n"], w["seed_base"], sizes=w["sizes"], k_range=w["k_range"])
+    t = synth.t_vector(0.0, w["t_end"], w["n_samples"])
+    return models, t
+
+
+def stratified(models, per_size):
+    idx = []
+    for S in WORKLOAD["sizes"]:
+        idx += [i for i, m in enumerate(models) if m.S == S][:per_size]
+    return idx
+
+
+def cpu_legs(models, t, sample, prefix_loops):
+    """Times the oracle and LSODA on the sample, one core, before any GPU work.
+    Returns per-instance records; the oracle's large instances run a prefix."""
+    from oracle import lsoda
+    from oracle import oracle as O
+    recs = []
+    for i in sample:
+        m = models[i]
+        full = m.S <= 128
+        kw = dict(CONTROLLER) if full else dict(CONTROLLER, max_iters=prefix_loops)
+        t0 = time.perf_counter()
+        _, st, rc, _, _ = O.solve(m, t, **kw)
+        t_orc = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        y_ls = lsoda.solve(m, t, rtol=1e-6)
+        t_ls = time.perf_counter() - t0
+        recs.append(dict(i=i, S=m.S, oracle_s=t_orc, oracle_full=full, oracle_bdf=int(st["bdf_steps"]),
+                         oracle_samples=int(st["samples"]), lsoda_s=t_ls, lsoda_out=y_ls))
+        say(f"  CPU S={m.S}: oracle {'full' if full else f'{prefix_loops}-iteration prefix'} {t_orc:.2f} s, "
+            f"LSODA {t_ls:.2f} s")
+    return recs
+
+
+def project(models, per_size_time):
+    """Workload time from per-size mean times: sum over sizes of count x mean."""
+    tot = 0.0
+    for S in WORKLOAD["sizes"]:
+        n = sum(1 for m in models if m.S == S)
+        tot += n * per_size_time[S]
+    return tot
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes first (at most one is run)")
+    ap.add_argument("--budget", type=float, default=420.0,
+                    help="wall-clock seconds for the whole run; timed steps stop when the next one would "
+                         "not fit, and the line says how many ran")
+    ap.add_argument("--plain-per-size", type=int, default=1, help="plain-path instances per size")
+    ap.add_argument("--cpu-per-size", type=int, default=1, help="CPU-baseline instances per size")
+    ap.add_argument("--cpu-prefix", type=int, default=1500,
+                    help="loop iterations of the oracle on S >= 256 sample instances")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--no-chosen", action="store_true", help="skip the batched-form comparison")
-    ap.add_argument("--batch", type=int, default=256, help="instances in the batched-form run")
-    ap.add_argument("--plain-subsample", type=int, default=8)
-    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form measurement")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the single-precision loop (-p float)")
-    ap.add_argument("--diff-batch", type=int, default=256)
-    ap.add_argument("--diff-t-end", type=float, default=0.2)
-    ap.add_argument("--precision-4096", type=int, default=0, metavar="N",
-                    help="also compare the plain and precision forms on the 4096x4096 model over "
-                         "its first N loop iterations (0: skip)")
+    ap.add_argument("--precision-batch", action="store_true",
+                    help="also time the batched precision form (FP32 LU + FP64 refinement) on the "
+                         "workload and report its error against the chosen form")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: solves are "
-              "independent, so several GPUs would each run their own share of models in separate "
-              "processes, with no collective in the solve path", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: instances are independent, "
+              "so several GPUs would each run their own share of the batch in separate processes, with no "
+              "collective in the solve path", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
-    from lassie_amd import _lib, synth
+    say("generating the 2048-instance workload")
+    models, t = workload()
+    cpu_sample = stratified(models, args.cpu_per_size)
+    cpu = None
+    if not args.no_cpu:
+        say(f"CPU baselines on {len(cpu_sample)} instances (one per size unless asked), one core")
+        cpu = cpu_legs(models, t, cpu_sample, args.cpu_prefix)
 
-    w = WORKLOAD
-    m = synth.stiff(w["S"], w["R"], w["seed"], w["k_range"])
-    t = synth.t_vector(0.0, w["t_end"], w["n_samples"])
-    say(f"plain FP64 solve, {w['S']}x{w['R']}, {args.steps} step(s) after {args.warmup} warmup")
-    dm = _lib.DeviceModel(m)
-    for _ in range(args.warmup):
-        dm.solve(t, **CONTROLLER)
-    kernel_ms, stats, out = [], None, None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out, _, stats, _, _ = dm.solve(t, **CONTROLLER)
-        kernel_ms.append(stats["kernel_ms"])
-    elapsed = time.perf_counter() - t0
-    dm.close()
-    k_ms = float(np.mean(kernel_ms))
-    fl = flops_per_solve(w["S"], stats)
-    achieved = fl / (k_ms / 1e3) / 1e12
+    from lassie_amd import _lib
+    say("uploading models")
+    dms = [_lib.DeviceModel(m) for m in models]
+    plan = _lib.BatchPlan(dms, t, **CONTROLLER)
+    variant, tile_rows, trial_ms = plan.info()
+    say(f"plan prepared (variant {variant}, trials {[round(x, 1) for x in trial_ms]} ms)")
+
+    finished = threading.Event()
+
+    def follow(t_run):
+        last, quiet = 0, 0
+        while not finished.wait(5.0):
+            done = plan.progress()
+            quiet += 1
+            if done != last or quiet >= 6:
+                last, quiet = done, 0
+                say(f"  {done} of {len(models)} instances finished ({time.perf_counter() - t_run:.1f} s)")
+
+    def run_once():
+        finished.clear()
+        t_run = time.perf_counter()
+        th = threading.Thread(target=follow, args=(t_run,), daemon=True)
+        th.start()
+        try:
+            res = plan.run(raise_on_maxiter=False)
+        finally:
+            finished.set()
+            th.join()
+        return res, time.perf_counter() - t_run
+
+    # One warmup pass at most: a pass is tens of seconds of deterministic
+    # work, so further warmups change nothing and the budget goes to timed
+    # passes instead.
+    plain_reserve = 45.0 * args.plain_per_size + 15.0
+    warm = 0
+    if args.warmup > 0:
+        say("warmup pass")
+        run_once()
+        warm = 1
+    say("timed passes")
+    walls, kernel_ms, res = [], [], None
+    while len(walls) < args.steps:
+        if walls and elapsed() + max(walls) + plain_reserve > args.budget:
+            say(f"budget: stopping after {len(walls)} timed pass(es)")
+            break
+        res, w = run_once()
+        walls.append(w)
+        kernel_ms.append(plan.timing()[2])
+        say(f"  pass {len(walls)}: {w:.2f} s")
+    outs, ys, stats, dev_ms, rc = res
+    n_wave, n_wg, _ = plan.timing()
+    steps = len(walls)
+    wall = float(np.sum(walls))
+    value = len(models) * steps / wall
+    capped = [i for i, s in enumerate(stats) if s["samples"] < t.size]
+
+    # roofline of the workgroup-level kernel, from its own events
+    wg_idx = [i for i, m in enumerate(models) if m.S > 64]
+    wv_idx = [i for i, m in enumerate(models) if m.S <= 64]
+    wg_flops = sum(work_flops(models[i].S, stats[i]) for i in wg_idx)
+    wv_flops = sum(work_flops(models[i].S, stats[i]) for i in wv_idx)
+    wg_ms = float(np.mean([k[1] for k in kernel_ms]))
+    wv_ms = float(np.mean([k[0] for k in kernel_ms]))
+    achieved = wg_flops / (wg_ms / 1e3) / 1e12
+
+    # plain path on a stratified sample of the same instances, bit-exact check
+    plain_idx = stratified(models, args.plain_per_size)
+    say(f"plain path on {len(plain_idx)} instances")
+    plain_t, mismatches = {}, 0
+    for i in plain_idx:
+        t0 = time.perf_counter()
+        out_p, y_p, st_p, rc_p, _ = dms[i].solve(t, raise_on_maxiter=False, **CONTROLLER)
+        dt = time.perf_counter() - t0
+        plain_t.setdefault(models[i].S, []).append(dt)
+        n = int(st_p["samples"])
+        same = (np.array_equal(out_p[:n], outs[i][:n]) and np.array_equal(y_p, ys[i])
+                and st_p["newton_iters"] == stats[i]["newton_iters"])
+        mismatches += 0 if same else 1
+        say(f"  plain S={models[i].S}: {dt:.2f} s, {'bit-identical' if same else 'DIFFERS'}")
+    plain_mean = {S: float(np.mean(v)) for S, v in plain_t.items()}
+    plain_rate = len(models) / project(models, plain_mean) if len(plain_mean) == len(WORKLOAD["sizes"]) else None
+
+    per_size = {}
+    for S in WORKLOAD["sizes"]:
+        idx = [i for i, m in enumerate(models) if m.S == S]
+        ph = np.array([stats[i]["phase_ms"] for i in idx])
+        per_size[str(S)] = {
+            "instances": len(idx), "capped": sum(1 for i in idx if i in capped),
+            "instance_ms_mean": float(ph.sum(1).mean()), "instance_ms_max": float(ph.sum(1).max()),
+            "lu_us": float(1e3 * ph[:, 2].sum() / max(1, sum(stats[i]["lu_factorizations"] for i in idx))),
+            "newton_iter_us": float(1e3 * ph[:, 3].sum() / max(1, sum(stats[i]["newton_iters"] for i in idx))),
+            "plain_s_mean": plain_mean.get(S)}
+
     line = {
-        "metric": METRIC, "value": args.steps / elapsed, "unit": "solves/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "1024x1024 stiff mass-action model, plain FP64 RKF45/BDF1, "
-                               "t=[0,100], 200 samples, rtol 1e-6",
-                   "form": "plain", "species": w["S"], "reactions": w["R"], "seed": w["seed"],
-                   "k_range": list(w["k_range"]), "controller": CONTROLLER},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "solve_kernel<4,16,double,false> (persistent: one launch = one solve)",
-                     "flops_per_launch": fl, "kernel_ms": k_ms,
-                     "phase_ms": {"control_rkf": stats["phase_ms"][0],
-                                  "jacobian": stats["phase_ms"][1], "lu": stats["phase_ms"][2],
-                                  "newton_solves": stats["phase_ms"][3]}},
-        "solver_stats": {k: stats[k] for k in ("loop_iters", "rkf_attempts", "rkf_accepts",
-                                                "bdf_steps", "newton_iters", "lu_factorizations",
-                                                "lu_failures", "nan_repairs", "groups")},
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": 1e3 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2048 uneven stiff mass-action instances (S in {16..512}, R = 2S, seeds "
+                               "1000..3047, k logU[1e-4, 1e4]), t = [0, 50], 100 samples, rtol 1e-6; "
+                               "batched FP64 plan vs plain FP64 single-model solves",
+                   "instances": len(models), "sizes": list(WORKLOAD["sizes"]), "controller": CONTROLLER,
+                   "form": "batched FP64 plan: workgroup-level kernel (S > 64) and wave-level kernel "
+                           "(S <= 64) concurrently, hipGraph replay"},
+        "chosen_vs_plain": {
+            "chosen_solves_per_s": value, "plain_solves_per_s_projected": plain_rate,
+            "speedup": value / plain_rate if plain_rate else None,
+            "max_rel_err_vs_plain": (0.0 if mismatches == 0 else None) if plain_idx else None,
+            "plain_sample": {"instances": len(plain_idx), "per_size": args.plain_per_size,
+                             "bit_identical": mismatches == 0,
+                             "note": "plain rate = 2048 / sum over sizes of (instances x the sample's mean "
+                                     "plain solve time)"}},
+        "capped": {"instances": len(capped), "max_newton": CONTROLLER["max_newton"],
+                   "note": "instances stopped by the per-instance Newton-iteration cap (same cap and same "
+                           "stopping point in every form and in the oracle)"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "lassie::batch_kernel (workgroup-level, S > 64 instances)",
+                     "flops_per_launch": wg_flops, "kernel_ms": wg_ms,
+                     "wave_kernel": {"instances": n_wave, "ms": wv_ms, "flops": wv_flops,
+                                     "tflops": wv_flops / (wv_ms / 1e3) / 1e12 if wv_ms > 0 else None}},
+        "per_size": per_size, "device_ms_per_step": dev_ms, "walls_s": walls,
     }
-    traffic = os.path.join(ROOT, "profiles", "traffic_solve_1024.json")
+    traffic = os.path.join(ROOT, "profiles", "traffic_batch_c4.json")
     if os.path.exists(traffic):
         with open(traffic) as f:
-            line["roofline"]["traffic"] = json.load(f).get("bytes_per_launch")
-    say(f"plain solve done: {elapsed:.2f} s")
-    if not args.no_cpu:
-        say("CPU LSODA baseline")
-        t_lsoda, err_max, err_med = cpu_lsoda(m, t, out)
-        proj, unit = cpu_same_algorithm(m, stats)
+            tr = json.load(f)
+        line["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        line["roofline"]["traffic_GBps"] = tr.get("bytes_per_launch") / (wg_ms / 1e3) / 1e9
+
+    if cpu is not None:
+        # oracle: full solves below 256 species; larger ones scaled from their
+        # prefix by the instance's full BDF-step count in the GPU run
+        orc = {}
+        for r in cpu:
+            full_s = r["oracle_s"] if r["oracle_full"] else \
+                r["oracle_s"] * stats[r["i"]]["bdf_steps"] / max(1, r["oracle_bdf"])
+            orc.setdefault(r["S"], []).append(full_s)
+        orc_mean = {S: float(np.mean(v)) for S, v in orc.items()}
+        orc_rate = len(models) / project(models, orc_mean)
+        lso_mean = {}
+        for r in cpu:
+            lso_mean.setdefault(r["S"], []).append(r["lsoda_s"])
+        lso_mean = {S: float(np.mean(v)) for S, v in lso_mean.items()}
+        lso_rate = len(models) / project(models, lso_mean)
+        errs = []
+        for r in cpu:
+            i = r["i"]
+            if stats[i]["samples"] == t.size:
+                y = r["lsoda_out"]
+                errs.append(np.abs(outs[i] - y) / (np.abs(y) + models[i].atol[None, :]))
+        allerr = np.concatenate([e.ravel() for e in errs]) if errs else np.zeros(1)
         line["cpu_baseline"] = {
-            "value": 1.0 / t_lsoda, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "one full solve of the same 1024x1024 model by the LSODA path (scipy odeint "
-                      "restating CPU-based LSODA/LSODAsim.py; rtol 1e-6, atol 1e-12, 1 thread)",
-            "seconds": t_lsoda}
-        line["cpu_same_algorithm"] = {
-            "value": 1.0 / proj, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle (CPU restatement of the same RKF45/BDF1 loop) per-operation cost on "
-                      "this model x the GPU run's operation counts", "projected_seconds": proj,
-            **unit}
-        line["max_rel_err_vs_lsoda"] = err_max
-        line["median_rel_err_vs_lsoda"] = err_med
-    if not args.no_fp32:
-        say("single-precision loop (-p float) on the same model")
-        dm = _lib.DeviceModel(m)
+            "value": orc_rate, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"the oracle (FP64 CPU restatement of the same RKF45/BDF1 loop, same controller and cap) "
+                      f"on the first {args.cpu_per_size} instance(s) of each size: full solves for S <= 128, "
+                      f"the first {args.cpu_prefix} loop iterations for S = 256 and 512 scaled by the "
+                      f"instance's full BDF-step count; workload rate = 2048 / sum of (instances x mean)",
+            "per_size_s": orc_mean}
+        line["cpu_lsoda"] = {
+            "value": lso_rate, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": "LSODA (scipy odeint, the reference's CPU method; rtol 1e-6, atol 1e-12) on the same "
+                      "instances, projected the same way; a different algorithm (variable-order BDF)",
+            "per_size_s": lso_mean,
+            "rel_err_vs_chosen_median": float(np.median(allerr)), "rel_err_vs_chosen_max": float(allerr.max())}
+
+    if args.precision_batch and elapsed() + 2 * (wall / steps) < args.budget:
+        say("batched precision form (FP32 LU + FP64 refinement)")
+        p32 = _lib.BatchPlan(dms, t, precision=_lib.PREC_FP32_REFINE, **CONTROLLER)
         t0 = time.perf_counter()
-        out32, _, st32, rc32, _ = dm.solve(t, precision=_lib.PREC_FP32_LOOP, **CONTROLLER)
-        el32 = time.perf_counter() - t0
+        o32, _, s32, _, _ = p32.run(raise_on_maxiter=False)
+        w32 = time.perf_counter() - t0
+        p32.close()
+        rel = [float((np.abs(a - b) / (np.abs(b) + m.atol[None, :])).max())
+               for a, b, m in zip(o32, outs, models)]
+        line["precision_batch"] = {"solves_per_s": len(models) / w32, "max_rel_err_vs_fp64": float(np.max(rel)),
+                                   "median_instance_max_rel_err": float(np.median(rel)),
+                                   "fallbacks": int(sum(s["refine_fallbacks"] for s in s32))}
+    plan.close()
+    for dm in dms:
         dm.close()
-        rel = np.abs(out32 - out) / (np.abs(out) + m.atol[None, :])
-        line["fp32_loop"] = {
-            "solves_per_s": 1.0 / el32, "kernel_ms": st32["kernel_ms"],
-            "lu_ms_per_factorization": st32["phase_ms"][2] / max(st32["lu_factorizations"], 1),
-            "bdf_steps": st32["bdf_steps"], "newton_iters": st32["newton_iters"],
-            "max_rel_err_vs_fp64": float(rel.max()), "median_rel_err_vs_fp64": float(np.median(rel)),
-            "note": "Simulation<float> semantics (the reference's -p float), bit-identical to the "
-                    "single-precision oracle"}
-        say(f"single-precision loop done: {el32:.2f} s")
-    if not args.no_chosen:
-        say(f"batched form, {args.batch} uneven instances; plain path on {args.plain_subsample}")
-        line["chosen_form"] = chosen_form(args.batch, args.plain_subsample, True, not args.no_cpu)
-    if not args.no_diff:
-        say(f"differentiation form, {args.diff_batch} parameterisations of 512x1024, t in [0, {args.diff_t_end}]")
-        line["differentiation_form"] = differentiation_form(args.diff_batch, args.diff_t_end)
-    if args.precision_4096 > 0:
-        say(f"precision form on 4096x4096, {args.precision_4096} loop iterations")
-        line["precision_form_4096"] = precision_form_4096(args.precision_4096)
+    line["bench_wall_s"] = elapsed()
     print(json.dumps(line), flush=True)
 
 

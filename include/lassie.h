@@ -151,8 +151,11 @@ int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lass
  * lassie_solve_batch; rates changed with lassie_model_set_rates are used by
  * the next run.  The models must outlive the plan.  lassie_batch_plan_info
  * reports the chosen variant, its row tiles and the trial times (ms, one per
- * variant; zeros when the choice came from the cache).  No counterpart in the
- * reference (one model per process). */
+ * variant; zeros when the choice came from the cache).  Choices persist in a
+ * JSON file ($LASSIE_TUNE_CACHE, else $XDG_CACHE_HOME/lassie/tune.json, else
+ * ~/.cache/lassie/tune.json; LASSIE_TUNE_CACHE=off keeps them in the process),
+ * keyed by device architecture, CU count, largest S, instance count and
+ * precision.  No counterpart in the reference (one model per process). */
 typedef struct lassie_batch_plan lassie_batch_plan;
 int lassie_batch_plan_create(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
                              const double *t, int32_t nt, lassie_batch_plan **plan);
@@ -167,10 +170,12 @@ int lassie_batch_plan_info(const lassie_batch_plan *plan, int32_t *variant, int3
  * ProgressSimulation file, Simulation.cuh:660-672). */
 int lassie_batch_plan_progress(const lassie_batch_plan *plan, int64_t *done);
 /* How the plan divides its instances between the batched form's two kernels
- * and their device times in the last run (ms[0]: the wave-level kernel, one
- * wavefront per instance of at most 64 species, FP64; ms[1]: the
- * workgroup-level kernel with the copies out).  Any output may be NULL.  No
- * counterpart in the reference (one model per process). */
+ * and their device times in the last run, each measured by events on its own
+ * stream (ms[0]: the wave-level kernel, one wavefront per instance of at most
+ * 64 species, FP64; ms[1]: the workgroup-level kernel).  The two kernels run
+ * concurrently: the workgroup-level one first, the wave-level one on the CUs
+ * it leaves free.  Any output may be NULL.  No counterpart in the reference
+ * (one model per process). */
 int lassie_batch_plan_timing(const lassie_batch_plan *plan, int32_t *n_wave, int32_t *n_wg, float *ms);
 int lassie_batch_plan_destroy(lassie_batch_plan *plan);
 

@@ -22,6 +22,7 @@
 #include "../../include/lassie.h"
 #include "batch.h"
 #include "model.h"
+#include "tune_cache.h"
 #include "wave.h"
 
 namespace {
@@ -45,11 +46,15 @@ struct lassie_batch_plan {
     double *h_out = nullptr, *h_y = nullptr, *h_dst = nullptr;
     long long *h_st = nullptr;
     unsigned int *h_done = nullptr;  // per slot: instances finished in the current run
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
-    // graph 0: queue reset + wave-level kernel; graph 1: workgroup-level kernel + copies out
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipStream_t stream = nullptr;   // queue reset, workgroup-level kernel, copies out
+    hipStream_t wstream = nullptr;  // wave-level kernel, concurrent with the workgroup-level one
+    // ev[0] run start, ev[1] queues reset, ev[2] workgroup-level kernel done,
+    // ev[3] wave-level kernel start, ev[4] its end, ev[5] run end
+    hipEvent_t ev[6] = {};
+    hipEvent_t &ev0 = ev[0], &ev1 = ev[5];
+    // graphs: 0 queue reset, 1 workgroup-level kernel, 2 wave-level kernel, 3 copies out
+    hipGraph_t graph[4] = {};
+    hipGraphExec_t exec[4] = {};
     float tune_ms[lassie::kBatchVariants] = {0};
     float last_ms[2] = {0.f, 0.f};  // device time of the last run's two kernels
     lassie::BatchArgs args{}, wargs{};
@@ -57,7 +62,7 @@ struct lassie_batch_plan {
 
 static void free_plan(lassie_batch_plan *p)
 {
-    for (int g = 0; g < 2; g++) {
+    for (int g = 0; g < 4; g++) {
         if (p->exec[g]) (void)hipGraphExecDestroy(p->exec[g]);
         if (p->graph[g]) (void)hipGraphDestroy(p->graph[g]);
     }
@@ -68,10 +73,10 @@ static void free_plan(lassie_batch_plan *p)
     void *hptrs[] = {p->h_out, p->h_y, p->h_dst, p->h_st, p->h_done};
     for (void *q : hptrs)
         if (q) (void)hipHostFree(q);
-    if (p->ev0) (void)hipEventDestroy(p->ev0);
-    if (p->evm) (void)hipEventDestroy(p->evm);
-    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    for (hipEvent_t e : p->ev)
+        if (e) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->wstream) (void)hipStreamDestroy(p->wstream);
     delete p;
 }
 
@@ -135,8 +140,10 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     const size_t ws32_stride = (size_t)smax * smax + lassie::kLuSink;
     const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
     hipError_t e;
-    if ((e = hipStreamCreate(&P->stream)) != hipSuccess || (e = hipEventCreate(&P->ev0)) != hipSuccess ||
-        (e = hipEventCreate(&P->evm)) != hipSuccess || (e = hipEventCreate(&P->ev1)) != hipSuccess ||
+    for (hipEvent_t &ev : P->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess)
+            return bail(LASSIE_ERR_HIP, std::string("batch plan: ") + hipGetErrorString(e));
+    if ((e = hipStreamCreate(&P->stream)) != hipSuccess || (e = hipStreamCreate(&P->wstream)) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_inst, sizeof(lassie::BatchInst) * n_models)) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_order, sizeof(int) * n_models)) != hipSuccess ||
         (e = hipMalloc((void **)&P->d_next, 2 * sizeof(int))) != hipSuccess ||
@@ -204,9 +211,22 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     aw.nh_max = sp.nh_max_wave;
     aw.done_host = P->h_done + slots;
 
-    // autotuner: a short trial (64 loop iterations per instance) of every
-    // variant; the fastest is kept for this (device, largest S, count, precision)
-    const auto key = std::make_tuple(P->device, smax, n_models, (int)o.precision);
+    // autotuner: a short trial of every kernel variant on the plan's own
+    // workgroup-level instances (16 loop iterations, at most 256 Newton
+    // iterations each); the fastest is kept for this device and workload
+    // shape, in this process and in the cache file (tune_cache.h)
+    const auto key = std::make_tuple(P->device, smax, sp.n_wg, (int)o.precision);
+    std::string fkey;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, P->device) == hipSuccess) {
+            std::string arch(prop.gcnArchName);
+            for (char &c : arch)
+                if (c == '"' || c == '\\') c = '_';
+            fkey = "batch_variant/" + arch + "/cu" + std::to_string(prop.multiProcessorCount) + "/smax" +
+                   std::to_string(smax) + "/n" + std::to_string(sp.n_wg) + "/prec" + std::to_string(o.precision);
+        }
+    }
     int chosen = -1;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -214,9 +234,13 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         if (it != g_tuned.end()) chosen = it->second;
     }
     if (chosen < 0 && sp.n_wg == 0) chosen = lassie::kBatchDefaultVariant;
+    if (chosen < 0 && !fkey.empty() && lassie::tune_lookup(fkey, &chosen) &&
+        (chosen < 0 || chosen >= lassie::kBatchVariants))
+        chosen = -1;
     if (chosen < 0) {
         lassie::BatchArgs trial = a;
-        trial.max_iters = 64;
+        trial.max_iters = 16;
+        trial.max_newton = 256;
         trial.done_host = nullptr;
         float best = 0.f;
         for (int v = 0; v < lassie::kBatchVariants; v++) {
@@ -234,36 +258,42 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
                 chosen = v;
             }
         }
+        if (!fkey.empty()) lassie::tune_store(fkey, chosen, P->tune_ms, lassie::kBatchVariants);
+    }
+    {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[key] = chosen;
     }
     P->variant = chosen;
     a.done_host = P->h_done;
 
-    // the launch sequence as two graphs (no memset nodes: the queue heads are
-    // reset by a kernel): [reset, wave-level kernel] and [workgroup-level
-    // kernel, copies out]; an event between them times each kernel
-    for (int g = 0; g < 2; g++) {
-        if ((e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+    // the launch sequence as four graphs (no memset nodes: the queue heads
+    // are reset by a kernel).  The workgroup-level kernel starts first and
+    // holds every CU it can; the wave-level kernel, on a second stream, takes
+    // CUs as the large instances' queue drains; events between the graphs
+    // time each kernel on its own stream.
+    for (int g = 0; g < 4; g++) {
+        hipStream_t s = (g == 2) ? P->wstream : P->stream;
+        if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return bail(LASSIE_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
         hipError_t ec = hipSuccess;
         if (g == 0) {
-            ec = lassie::launch_batch_reset(P->d_next, P->stream);
-            if (ec == hipSuccess && sp.n_wave) ec = lassie::launch_wave(aw, P->wave_blocks, sp.wave_lds, P->stream);
+            ec = lassie::launch_batch_reset(P->d_next, s);
+        } else if (g == 1) {
+            if (sp.n_wg) ec = lassie::launch_batch(a, slots, s, P->variant);
+        } else if (g == 2) {
+            if (sp.n_wave) ec = lassie::launch_wave(aw, P->wave_blocks, sp.wave_lds, s);
         } else {
-            if (sp.n_wg) ec = lassie::launch_batch(a, slots, P->stream, P->variant);
+            ec = hipMemcpyAsync(P->h_out, P->d_out, sizeof(double) * P->out_tot, hipMemcpyDeviceToHost, s);
             if (ec == hipSuccess)
-                ec = hipMemcpyAsync(P->h_out, P->d_out, sizeof(double) * P->out_tot, hipMemcpyDeviceToHost, P->stream);
-            if (ec == hipSuccess)
-                ec = hipMemcpyAsync(P->h_y, P->d_y, sizeof(double) * P->y_tot, hipMemcpyDeviceToHost, P->stream);
+                ec = hipMemcpyAsync(P->h_y, P->d_y, sizeof(double) * P->y_tot, hipMemcpyDeviceToHost, s);
             if (ec == hipSuccess)
                 ec = hipMemcpyAsync(P->h_st, P->d_st, sizeof(long long) * lassie::kBatchStats * n_models,
-                                    hipMemcpyDeviceToHost, P->stream);
+                                    hipMemcpyDeviceToHost, s);
             if (ec == hipSuccess)
-                ec = hipMemcpyAsync(P->h_dst, P->d_dst, sizeof(double) * 4 * n_models, hipMemcpyDeviceToHost,
-                                    P->stream);
+                ec = hipMemcpyAsync(P->h_dst, P->d_dst, sizeof(double) * 4 * n_models, hipMemcpyDeviceToHost, s);
         }
-        e = hipStreamEndCapture(P->stream, &P->graph[g]);
+        e = hipStreamEndCapture(s, &P->graph[g]);
         if (ec != hipSuccess || e != hipSuccess)
             return bail(LASSIE_ERR_HIP,
                         std::string("graph capture: ") + hipGetErrorString(ec != hipSuccess ? ec : e));
@@ -282,16 +312,19 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
     hipError_t e;
     // no launch of this plan is in flight here (runs are synchronous)
     std::memset(P->h_done, 0, sizeof(unsigned int) * (P->slots + P->wave_blocks));
-    if ((e = hipSetDevice(P->device)) != hipSuccess || (e = hipEventRecord(P->ev0, P->stream)) != hipSuccess ||
-        (e = hipGraphLaunch(P->exec[0], P->stream)) != hipSuccess ||
-        (e = hipEventRecord(P->evm, P->stream)) != hipSuccess ||
-        (e = hipGraphLaunch(P->exec[1], P->stream)) != hipSuccess ||
-        (e = hipEventRecord(P->ev1, P->stream)) != hipSuccess || (e = hipStreamSynchronize(P->stream)) != hipSuccess)
+    hipStream_t s0 = P->stream, s1 = P->wstream;
+    if ((e = hipSetDevice(P->device)) != hipSuccess || (e = hipEventRecord(P->ev[0], s0)) != hipSuccess ||
+        (e = hipGraphLaunch(P->exec[0], s0)) != hipSuccess || (e = hipEventRecord(P->ev[1], s0)) != hipSuccess ||
+        (e = hipGraphLaunch(P->exec[1], s0)) != hipSuccess || (e = hipEventRecord(P->ev[2], s0)) != hipSuccess ||
+        (e = hipStreamWaitEvent(s1, P->ev[1], 0)) != hipSuccess || (e = hipEventRecord(P->ev[3], s1)) != hipSuccess ||
+        (e = hipGraphLaunch(P->exec[2], s1)) != hipSuccess || (e = hipEventRecord(P->ev[4], s1)) != hipSuccess ||
+        (e = hipStreamWaitEvent(s0, P->ev[4], 0)) != hipSuccess || (e = hipGraphLaunch(P->exec[3], s0)) != hipSuccess ||
+        (e = hipEventRecord(P->ev[5], s0)) != hipSuccess || (e = hipStreamSynchronize(s0)) != hipSuccess)
         return api_fail(LASSIE_ERR_HIP, std::string("plan run: ") + hipGetErrorString(e));
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, P->ev0, P->ev1);
-    (void)hipEventElapsedTime(&P->last_ms[0], P->ev0, P->evm);
-    (void)hipEventElapsedTime(&P->last_ms[1], P->evm, P->ev1);
+    (void)hipEventElapsedTime(&ms, P->ev[0], P->ev[5]);
+    (void)hipEventElapsedTime(&P->last_ms[0], P->ev[3], P->ev[4]);
+    (void)hipEventElapsedTime(&P->last_ms[1], P->ev[1], P->ev[2]);
     if (elapsed_ms) *elapsed_ms = ms;
     std::memcpy(out, P->h_out, sizeof(double) * P->out_tot);
     if (y_final) std::memcpy(y_final, P->h_y, sizeof(double) * P->y_tot);

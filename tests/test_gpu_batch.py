@@ -143,3 +143,43 @@ def test_batch_plan_replays_the_batched_form(gpu_lib):
     o_out, _, o_rc, _, _ = O.solve(m3, t, **kw)
     assert np.array_equal(outs[3], o_out)
     plan.close()
+
+
+_TUNE_CHILD = r'''
+import json, sys
+sys.path.insert(0, sys.argv[1])
+from lassie_amd import _lib, synth
+models = synth.uneven_batch(6, 4444, sizes=(100, 200))
+t = synth.t_vector(0, 1, 5)
+dms = [_lib.DeviceModel(m) for m in models]
+plan = _lib.BatchPlan(dms, t, rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
+print(json.dumps(plan.info()))
+plan.close()
+'''
+
+
+def test_plan_autotuner_choice_persists(gpu_lib, tmp_path):
+    """The autotuner stores its choice in the cache file (valid JSON, keyed by
+    device and workload shape); a new process with the same file reuses it
+    and runs no trials (trial times all zero)."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cache = tmp_path / "tune.json"
+    env = dict(os.environ, LASSIE_TUNE_CACHE=str(cache))
+    runs = []
+    for _ in range(2):
+        print("[plan] autotuner child process", flush=True)
+        r = subprocess.run([sys.executable, "-c", _TUNE_CHILD, root], env=env, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        runs.append(json.loads(r.stdout.strip().splitlines()[-1]))
+    (v1, _, ms1), (v2, _, ms2) = runs
+    assert max(ms1) > 0 and ms2 == [0.0, 0.0, 0.0] and v1 == v2
+    doc = json.loads(cache.read_text())
+    assert doc["format"] == "lassie-tune-1"
+    (key, entry), = doc["entries"].items()
+    assert key.startswith("batch_variant/") and "/smax200/n6/prec0" in key
+    assert entry["choice"] == v1 and len(entry["trial_ms"]) == 3
