@@ -62,12 +62,25 @@ def build():
 _lib = None
 
 
-def lib():
+def use_library(path):
+    """Loads the oracle from another build of the same sources (the tests
+    that plant a mistake in a copy of the oracle use it); None restores the
+    default build.  Returns the previously loaded library handle."""
+    global _lib
+    prev = _lib
+    _lib = None
+    if path is not None:
+        lib(path)
+    return prev
+
+
+def lib(path=None):
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
+        path = path or _LIB
+        if path == _LIB and not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path)
         L.orc_rhs.argtypes = [C.POINTER(OrcModel), _f64p, _f64p]
         L.orc_jac.argtypes = [C.POINTER(OrcModel), _f64p, _f64p]
         L.orc_newton_matrix.argtypes = [C.c_int, C.c_double, _f64p]
