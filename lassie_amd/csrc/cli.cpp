@@ -8,6 +8,7 @@
 #include <sys/stat.h>
 #include <sys/types.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -18,6 +19,8 @@
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime.h>
 
 #include "../../include/lassie.h"
 
@@ -104,6 +107,45 @@ bool is_dir(const string &p)
     std::cout << msg;
     std::cout.flush();
     exit(code);
+}
+
+// The reference's launch rule and its record in infoBlockThread
+// (returnBlockThread / minThread, Simulation.cuh:1225-1277, called for the
+// ODE, Jacobian and sampling launches at :583-598): `thread`
+// work items over `sm` compute units, at most 256 per block, rounded up to a
+// multiple of 32; appends (save 1) or starts (save 0) the file.  Returns
+// {blocks, threads per block}.  This build launches its own shapes; the
+// record keeps the reference's file and its verbose report.
+struct BlockThread {
+    int blocks, tpb;
+};
+
+BlockThread block_thread(int sm, int thread, const string &file, const string &type, int save)
+{
+    FILE *fd = fopen(file.c_str(), save == 0 ? "w" : "a");
+    if (fd) fprintf(fd, save == 0 ? "%s \n" : "\n\n%s \n", type.c_str());
+    if (fd) fprintf(fd, "Allocating %d threads over %d available streaming multiprocessors\n", thread, sm);
+    const float threadF = (float)thread;
+    const int c = (int)ceilf(threadF / (float)sm);
+    const int putative = 256 < c ? 256 : c;
+    if (fd) fprintf(fd, "Putative TPB: %d \n", putative);
+    float used_blocks, final_tpb;
+    if (putative % 32 == 0) {
+        final_tpb = (float)putative;
+        used_blocks = ceilf((float)thread / final_tpb);
+    } else {
+        if (fd) fprintf(fd, "Adapting to warp size\n");
+        final_tpb = (float)((putative / 32 + 1) * 32);
+        used_blocks = ceilf((float)thread / final_tpb);
+    }
+    BlockThread bt{(int)used_blocks, (int)final_tpb};
+    if (fd) {
+        fprintf(fd, "Final TPB: %d \n", bt.tpb);
+        fprintf(fd, "Number of used blocks: %d \n", bt.blocks);
+        fprintf(fd, "Total threads spawned: %d \n", (int)(final_tpb * used_blocks));
+        fclose(fd);
+    }
+    return bt;
 }
 
 }  // namespace
@@ -300,11 +342,15 @@ int main(int argc, char *argv[])
             die(-8, string("\nError: the modelkind is stochastic/number, but there is not volume file\n") + kRule +
                         "\n");
         const double conv = volume * avo;
+        if (verbose > 1)
+            std::cout << "\n\n" << kRule << "Convertion initial conditions and feeds with volume: " << volume
+                      << std::endl;
         for (int i = 0; i < S; i++) {
             x0[i] = x0[i] / conv;
             mfeed[i] = mfeed[i] / conv;
         }
         for (int r = 0; r < R; r++) {
+            if (r == 0 && verbose) std::cout << "Convertion reaction rate constants with volume: " << volume << std::endl;
             int distinct = 0;
             bool homo = false;
             for (int s = 0; s < S; s++) {
@@ -321,7 +367,6 @@ int main(int argc, char *argv[])
                 k[r] = k[r] * conv;
         }
     }
-    if (verbose > 1) std::cout << "\nThe model has " << S << " species and " << R << " reactions\n";
 
     // ---- compile and upload -----------------------------------------------------
     vector<int16_t> Ld((size_t)R * S), Rd((size_t)R * S);
@@ -367,6 +412,41 @@ int main(int argc, char *argv[])
         fclose(fd);
     }
 
+    // launch records (Simulation.cuh:583-620)
+    int sm = 0;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess) sm = prop.multiProcessorCount;
+    }
+    if (sm < 1) {
+        fprintf(stderr, "Error: no HIP device\n");
+        exit(-13);
+    }
+    const string info_file = outdir + "/infoBlockThread";
+    const BlockThread bt_ode = block_thread(sm, S, info_file, "ODE", 0);
+    const BlockThread bt_jac = block_thread(sm, S * S, info_file, "JAC", 1);
+    block_thread(sm, (int)cs.size(), info_file, "DINAM-SUBSPEC", 1);
+    if (verbose > 1) {
+        std::cout << kRule;
+        std::cout << "THREAD INFORMATIONS:\n";
+        std::cout << "Number of blocks: " << bt_ode.blocks << std::endl;
+        std::cout << "Number of threads: " << bt_ode.tpb << std::endl;
+        std::cout << "Number of Jacobian blocks: " << bt_jac.blocks << std::endl;
+        std::cout << "Number of Jacobian threads: " << bt_jac.tpb << std::endl;
+        // print(), Simulation.cuh:1135-1184
+        std::cout << "\n" << "***************************************************************";
+        std::cout << "\nThe model has " << S << " species and " << R << " reactions" << "\n";
+        if (!modelkind) {
+            std::cout << "\n" << kRule << "modelkind = deterministic" << "\n";
+        } else {
+            std::cout << "\n" << kRule << "modelkind = stochastic" << "\n";
+            std::cout << "\n" << kRule << "volume = " << volume << "\n";
+        }
+        std::cout << "\n\n###############################################################";
+        std::cout << "\nStart ode solver\n\n";
+        std::cout << "****Start simulation***" << std::endl;
+    }
+
     lassie_model_desc d;
     d.n_species = S;
     d.n_reactions = R;
@@ -397,11 +477,29 @@ int main(int argc, char *argv[])
     if (fp32) o.precision = LASSIE_PREC_FP32_LOOP;
     const int nt = (int)t.size(), ncs = (int)cs.size();
     vector<double> dinamic((size_t)nt * ncs);
-    const int64_t prog_cap = 1 << 22;
+    // ProgressSimulation holds one line per loop iteration.  The solve keeps
+    // the first prog_cap values; a longer run is solved again with room for
+    // every iteration (the solve is deterministic, so the second run repeats
+    // the first one), and a run too long to hold is an error.
+    // (LASSIE_PROGRESS_BUFFER sets the first buffer's lines)
+    int64_t prog_cap = 1 << 22;
+    if (const char *pb = getenv("LASSIE_PROGRESS_BUFFER")) prog_cap = std::max<int64_t>(1, atoll(pb));
+    const int64_t prog_max = (int64_t)1 << 28;
     vector<double> progress(prog_cap);
     lassie_stats st;
-    const int rc = lassie_solve(model, &o, t.data(), nt, cs.data(), ncs, dinamic.data(), nullptr, progress.data(),
-                                prog_cap, &st);
+    int rc = lassie_solve(model, &o, t.data(), nt, cs.data(), ncs, dinamic.data(), nullptr, progress.data(), prog_cap,
+                          &st);
+    if (rc == LASSIE_OK && st.loop_iters > prog_cap) {
+        if (st.loop_iters > prog_max) {
+            fprintf(stderr, "Error: %lld loop iterations exceed the %lld lines ProgressSimulation can hold\n",
+                    (long long)st.loop_iters, (long long)prog_max);
+            exit(-14);
+        }
+        prog_cap = st.loop_iters;
+        progress.assign(prog_cap, 0.0);
+        rc = lassie_solve(model, &o, t.data(), nt, cs.data(), ncs, dinamic.data(), nullptr, progress.data(), prog_cap,
+                          &st);
+    }
     lassie_model_destroy(model);
     if (rc == LASSIE_ERR_MAXITER) {
         fprintf(stderr, "Error: %s\n", lassie_last_error());
@@ -412,6 +510,19 @@ int main(int argc, char *argv[])
         exit(-13);
     }
 
+    if (verbose > 1) {
+        // the quarter marks the loop prints as samples are stored (Simulation.cuh:679-687)
+        for (int64_t i = 1; i <= st.samples; i++) {
+            if (i == nt / 4)
+                std::cout << "****1/4 of simulation***" << std::endl;
+            else if (i == nt / 2)
+                std::cout << "****1/2 of simulation***" << std::endl;
+            else if (i == 3 * nt / 4)
+                std::cout << "****3/4 of simulation***" << std::endl;
+        }
+        std::cout << "****End simulation***" << std::endl;
+    }
+
     // ProgressSimulation (Simulation.cuh:660-672): t/t_max per loop iteration
     {
         FILE *fd = fopen((outdir + "/ProgressSimulation").c_str(), "w");
@@ -419,19 +530,8 @@ int main(int argc, char *argv[])
             perror("Error open output file ProgressSimulation");
             exit(-11);
         }
-        const int64_t np = std::min<int64_t>(st.loop_iters, prog_cap);
-        for (int64_t i = 0; i < np; i++) fprintf(fd, "%.4e\n", progress[i]);
+        for (int64_t i = 0; i < st.loop_iters; i++) fprintf(fd, "%.4e\n", progress[i]);
         fclose(fd);
-    }
-    // infoBlockThread: the launch this solve used
-    {
-        FILE *fd = fopen((outdir + "/infoBlockThread").c_str(), "w");
-        if (fd) {
-            fprintf(fd, "ODE \nWorkgroups cooperating on the solve: %d of 256 threads\n", st.groups);
-            fprintf(fd, "Control state in LDS: %s\n", st.lds_model ? "yes" : "no");
-            fprintf(fd, "Device time of the integration loop: %.3f ms\n", st.kernel_ms);
-            fclose(fd);
-        }
     }
     // Solution (Simulation.cuh:942-971)
     {
@@ -448,6 +548,10 @@ int main(int argc, char *argv[])
             fprintf(fd, "\n");
         }
         fclose(fd);
+    }
+    if (verbose > 1) {
+        std::cout << "\nEnd ode solver\n";
+        std::cout << "###############################################################\n";
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     std::cout << "\n" << kRule;

@@ -187,3 +187,86 @@ def test_fed_species_print_no_ode_and_zero_jacobian_columns(tmp_path):
     i = lines.index("ODEs:")
     assert not any(l.startswith("dx[1] =") for l in lines[i + 1:i + 4])  # species 1 is fed
     assert all(f"dx[{r}][1] = 0" in lines for r in range(3))
+
+
+def block_thread_text(sm, thread, kind, save):
+    """infoBlockThread section of the reference's launch rule
+    (returnBlockThread, Simulation.cuh:1225-1277): threads per block =
+    min(256, ceil(thread / sm)) rounded up to a multiple of 32, in float."""
+    s = (f"{kind} \n" if save == 0 else f"\n\n{kind} \n")
+    s += f"Allocating {thread} threads over {sm} available streaming multiprocessors\n"
+    put = min(256, int(np.ceil(np.float32(thread) / np.float32(sm))))
+    s += f"Putative TPB: {put} \n"
+    tpb = put
+    if put % 32:
+        s += "Adapting to warp size\n"
+        tpb = (put // 32 + 1) * 32
+    blocks = int(np.ceil(np.float32(thread) / np.float32(tpb)))
+    return s + f"Final TPB: {tpb} \nNumber of used blocks: {blocks} \nTotal threads spawned: {tpb * blocks} \n"
+
+
+def test_block_thread_rule_known_answers():
+    """Hand-worked cases of the rule on 256 compute units: 32 species (ODE),
+    their 1024 Jacobian entries, 3 sampled species, and 4096^2 entries."""
+    assert block_thread_text(256, 32, "ODE", 0) == (
+        "ODE \nAllocating 32 threads over 256 available streaming multiprocessors\nPutative TPB: 1 \n"
+        "Adapting to warp size\nFinal TPB: 32 \nNumber of used blocks: 1 \nTotal threads spawned: 32 \n")
+    assert block_thread_text(256, 1024, "JAC", 1) == (
+        "\n\nJAC \nAllocating 1024 threads over 256 available streaming multiprocessors\nPutative TPB: 4 \n"
+        "Adapting to warp size\nFinal TPB: 32 \nNumber of used blocks: 32 \nTotal threads spawned: 1024 \n")
+    assert block_thread_text(256, 4096 * 4096, "JAC", 1).endswith(
+        "Putative TPB: 256 \nFinal TPB: 256 \nNumber of used blocks: 65536 \nTotal threads spawned: 16777216 \n")
+    assert "Adapting" not in block_thread_text(256, 8192, "ODE", 0)  # 32 per block exactly
+
+
+@pytest.mark.gpu
+def test_verbose_two_reports_launch_and_progress(tmp_path, gpu_lib):
+    """-v2: the THREAD INFORMATIONS block, the model summary, the start and
+    end banners and the quarter marks as samples are stored, in the
+    reference's order (Simulation.cuh:602-620, 655-657, 679-687, 934-937,
+    973-977); infoBlockThread holds the ODE, Jacobian and sampling records."""
+    m = synth.small_nonstiff(0)
+    t = synth.t_vector(0, 2, 8)
+    d = model_folder(tmp_path, m, t=t, cs=[0, 5, 31])
+    print("[cli] 32x48 model, -v2", flush=True)
+    rc, out = run(str(d), str(tmp_path / "out"), "-v2")
+    assert rc == 0, out
+    info = (tmp_path / "out" / "output" / "infoBlockThread").read_text()
+    sm = int(info.split("threads over ")[1].split()[0])
+    assert info == (block_thread_text(sm, 32, "ODE", 0) + block_thread_text(sm, 32 * 32, "JAC", 1)
+                    + block_thread_text(sm, 3, "DINAM-SUBSPEC", 1))
+    ode = block_thread_text(sm, 32, "ODE", 0).splitlines()
+    jac = block_thread_text(sm, 1024, "JAC", 1).splitlines()
+    blocks = lambda lines: int(lines[-2].split(":")[1])
+    tpb = lambda lines: int(lines[-3].split(":")[1])
+    expect = ["THREAD INFORMATIONS:", f"Number of blocks: {blocks(ode)}", f"Number of threads: {tpb(ode)}",
+              f"Number of Jacobian blocks: {blocks(jac)}", f"Number of Jacobian threads: {tpb(jac)}", "",
+              "***************************************************************",
+              "The model has 32 species and 48 reactions", "",
+              "***************************************************************", "modelkind = deterministic",
+              "", "", "###############################################################", "Start ode solver", "",
+              "****Start simulation***", "****1/4 of simulation***", "****1/2 of simulation***",
+              "****3/4 of simulation***", "****End simulation***", "", "End ode solver",
+              "###############################################################"]
+    lines = out.splitlines()
+    i = lines.index("THREAD INFORMATIONS:")
+    assert lines[i:i + len(expect)] == expect
+
+
+@pytest.mark.gpu
+def test_progress_file_longer_than_the_first_buffer(tmp_path, gpu_lib):
+    """A run with more loop iterations than the CLI's first progress buffer
+    (set to 50 lines here) writes every line: the solve is repeated with room
+    for all of them, and the file equals the oracle's log."""
+    m = synth.small_nonstiff(0)
+    t = synth.t_vector(0, 10, 100)
+    d = model_folder(tmp_path, m, t=t, cs=[0, 5, 31])
+    print("[cli] 32x48 model, 50-line first progress buffer", flush=True)
+    p = subprocess.run([EXE, str(d), str(tmp_path / "out")], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, LASSIE_PROGRESS_BUFFER="50"))
+    assert p.returncode == 0, p.stdout + p.stderr
+    o_out, o_st, _, prog, _ = O.solve(m, t, cs=np.array([0, 5, 31]), progress_cap=1 << 20)
+    assert o_st["loop_iters"] > 50
+    lines = (tmp_path / "out" / "output" / "ProgressSimulation").read_text().splitlines()
+    assert lines == ["%.4e" % v for v in prog]
+    assert (tmp_path / "out" / "output" / "Solution").read_text() == _fmt(t, o_out)
