@@ -38,6 +38,8 @@ extern "C" {
 #define LASSIE_ERR_UNSUPPORTED 5
 #define LASSIE_ERR_DEVICE 6    /* a device-side wait exceeded its bound */
 #define LASSIE_ERR_IO 7        /* file could not be read or written */
+#define LASSIE_ERR_NONFINITE 8 /* a computed result is not finite; the outputs are written (see
+                                * lassie_adjoint_status) */
 
 /* Arithmetic of the Newton linear solve. */
 #define LASSIE_PREC_FP64 0          /* plain method: FP64 LU (cuSOLVER getrf/getrs seat) */
@@ -208,6 +210,16 @@ int lassie_forward_batch(lassie_model *const *models, int32_t n_models, const la
  * and Newton iteration counts held fixed). */
 int lassie_adjoint_batch(lassie_tape *tape, const double *grad_out, double *grad_log_k, double *grad_x0,
                          double *elapsed_ms);
+/* Returns LASSIE_ERR_NONFINITE (every output written) when the gradient of
+ * at least one model is not finite.  That happens when a BDF1 step's
+ * modified Newton iteration ran to opts.newton_iter without converging
+ * (Simulation.cuh:789, 883-890: the reference keeps whatever iterate it
+ * reached): the derivative of such an iterate grows geometrically with the
+ * iterations and overflows.  lassie_adjoint_status reports, per model,
+ * whether its last gradient was finite (1), not finite (0) or not computed
+ * yet (-1), and how many of its recorded BDF1 steps ran the full
+ * opts.newton_iter iterations.  Either output may be NULL. */
+int lassie_adjoint_status(const lassie_tape *tape, int32_t model, int32_t *finite, int32_t *capped_newton_steps);
 
 /* Steps recorded for one model and the largest Newton iteration count of its
  * BDF1 steps; lassie_tape_read copies the record out: dt[n_steps], kind[n_steps]

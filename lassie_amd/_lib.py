@@ -23,6 +23,7 @@ LASSIE_ERR_MAXITER = 4
 LASSIE_ERR_UNSUPPORTED = 5
 LASSIE_ERR_DEVICE = 6
 LASSIE_ERR_IO = 7
+LASSIE_ERR_NONFINITE = 8
 
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
@@ -68,7 +69,7 @@ class Stats(C.Structure):
 # every symbol include/lassie.h declares
 EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy", "lassie_solve",
            "lassie_solve_batch", "lassie_model_set_rates", "lassie_forward_batch",
-           "lassie_adjoint_batch", "lassie_tape_info", "lassie_tape_read", "lassie_tape_destroy",
+           "lassie_adjoint_batch", "lassie_adjoint_status", "lassie_tape_info", "lassie_tape_read", "lassie_tape_destroy",
            "lassie_batch_plan_create", "lassie_batch_plan_run", "lassie_batch_plan_info",
            "lassie_batch_plan_progress", "lassie_batch_plan_timing", "lassie_batch_plan_destroy",
            "lassie_csr_from_dense", "lassie_device_count", "lassie_last_error", "lassie_abi_version")
@@ -107,6 +108,7 @@ def lib():
                                            C.POINTER(C.c_double), C.POINTER(C.c_void_p)]
         L.lassie_adjoint_batch.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, C.POINTER(C.c_double)]
         L.lassie_tape_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), _i32p]
+        L.lassie_adjoint_status.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p]
         L.lassie_tape_read.argtypes = [C.c_void_p, C.c_int32, _f64p, _i32p, _i32p]
         L.lassie_tape_destroy.argtypes = [C.c_void_p]
         L.lassie_batch_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts), _f64p,
@@ -286,14 +288,27 @@ class Tape:
         _check(lib().lassie_tape_read(self.h, i, _p(dt, _f64p), _p(kind, _i32p), _p(ss, _i32p)))
         return dt[:n], kind[:n], ss
 
-    def adjoint(self, grad_outs):
+    def status(self, i):
+        """(finite, capped_newton_steps) of model i: whether its last gradient
+        was finite (1), not finite (0) or not computed yet (-1), and how many
+        of its BDF1 steps ran the full newton_iter Newton iterations."""
+        f = np.zeros(1, np.int32)
+        c = np.zeros(1, np.int32)
+        _check(lib().lassie_adjoint_status(self.h, i, _p(f, _i32p), _p(c, _i32p)))
+        return int(f[0]), int(c[0])
+
+    def adjoint(self, grad_outs, allow_nonfinite=False):
         """grad_outs: list of dL/d(out_i) [nt, S_i].  Returns (list of dL/dlog k
-        [R_i], list of dL/dx0 [S_i], device ms)."""
+        [R_i], list of dL/dx0 [S_i], device ms).  A gradient that is not
+        finite (a BDF1 step whose modified Newton iteration did not converge)
+        raises LassieError unless allow_nonfinite; status(i) tells which."""
         g = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).ravel() for x in grad_outs]))
         gk = np.zeros(sum(m.R for m in self.models))
         gx = np.zeros(sum(m.S for m in self.models))
         ms = C.c_double(0.0)
-        _check(lib().lassie_adjoint_batch(self.h, _p(g, _f64p), _p(gk, _f64p), _p(gx, _f64p), C.byref(ms)))
+        rc = lib().lassie_adjoint_batch(self.h, _p(g, _f64p), _p(gk, _f64p), _p(gx, _f64p), C.byref(ms))
+        if rc != LASSIE_OK and not (rc == LASSIE_ERR_NONFINITE and allow_nonfinite):
+            _check(rc)
         gks, gxs = [], []
         ko = xo = 0
         for m in self.models:

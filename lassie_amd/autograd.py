@@ -4,7 +4,10 @@
 one reaction network (B uploaded models with the same species and reactions)
 with rate constants exp(log_k[b]) and returns their sampled trajectories
 [B, nt, S]; backpropagating through it runs the discrete adjoint
-(lassie_adjoint_batch) through the recorded RKF45/BDF1 steps.
+(lassie_adjoint_batch) through the recorded RKF45/BDF1 steps.  A
+parameterisation whose trajectory contains a BDF1 step whose modified Newton
+iteration did not converge has no finite gradient; backward then raises
+LassieError (LASSIE_ERR_NONFINITE) rather than hand NaNs to an optimizer.
 
 Tensors cross the boundary as FP64 host arrays: device memory belongs to the
 library (the forward pass, its step record and the reverse pass all stay on
@@ -39,8 +42,10 @@ def _function():
         @staticmethod
         def backward(ctx, grad_out):
             g = grad_out.detach().cpu().numpy().astype(np.float64)
-            gks, _, _ = ctx.tape.adjoint(list(g))
-            ctx.tape.close()
+            try:
+                gks, _, _ = ctx.tape.adjoint(list(g))
+            finally:
+                ctx.tape.close()
             return torch.from_numpy(np.stack(gks)), None, None, None, None
 
     return LassieTrajectories

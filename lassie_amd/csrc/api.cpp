@@ -667,6 +667,7 @@ struct lassie_tape {
     int nt = 0;
     long long cap = 0;  // steps per instance
     std::vector<long long> n_steps, max_kind, tape_off, ckpt_off, out_off;
+    std::vector<int> grad_finite;  // per model: last gradient finite (1), not (0), none yet (-1)
     long long out_tot = 0;
     double *d_dt = nullptr, *d_ckpt = nullptr;
     int *d_kind = nullptr, *d_sample = nullptr;
@@ -912,6 +913,7 @@ extern "C" int lassie_forward_batch(lassie_model *const *models, int32_t n_model
     tp->cap = max_steps;
     tp->n_steps.assign(n_models, 0);
     tp->max_kind.assign(n_models, 0);
+    tp->grad_finite.assign(n_models, -1);
     tp->tape_off.assign(n_models, 0);
     tp->ckpt_off.assign(n_models, 0);
     tp->out_off.assign(n_models, 0);
@@ -944,6 +946,24 @@ extern "C" int lassie_tape_read(const lassie_tape *tp, int32_t model, double *dt
     if (sample_step)
         HIP_TRY(hipMemcpy(sample_step, tp->d_sample + (size_t)model * tp->nt, sizeof(int) * tp->nt,
                           hipMemcpyDeviceToHost));
+    return LASSIE_OK;
+}
+
+extern "C" int lassie_adjoint_status(const lassie_tape *tp, int32_t model, int32_t *finite,
+                                     int32_t *capped_newton_steps)
+{
+    if (!tp || model < 0 || model >= (int)tp->models.size()) return fail(LASSIE_ERR_INVALID, "bad tape or model index");
+    if (finite) *finite = tp->grad_finite[model];
+    if (capped_newton_steps) {
+        *capped_newton_steps = 0;
+        const long long n = tp->n_steps[model];
+        if (n && tp->max_kind[model] >= tp->opts.newton_iter) {
+            std::vector<int> kind(n);
+            HIP_TRY(hipSetDevice(tp->device));
+            HIP_TRY(hipMemcpy(kind.data(), tp->d_kind + tp->tape_off[model], sizeof(int) * n, hipMemcpyDeviceToHost));
+            for (int k : kind) *capped_newton_steps += (k >= tp->opts.newton_iter) ? 1 : 0;
+        }
+    }
     return LASSIE_OK;
 }
 
@@ -1083,6 +1103,19 @@ extern "C" int lassie_adjoint_batch(lassie_tape *tp, const double *grad_out, dou
     cleanup();
     for (int i = 0; i < n; i++)
         if (status[i]) return fail(LASSIE_ERR_INVALID, "an LU that succeeded in the forward pass failed in the replay");
+    int n_bad = 0;
+    for (int i = 0; i < n; i++) {
+        bool ok = true;
+        for (long long r = 0; r < inst[i].R && ok; r++) ok = std::isfinite(grad_log_k[inst[i].gk_off + r]);
+        if (grad_x0)
+            for (long long s = 0; s < inst[i].S && ok; s++) ok = std::isfinite(grad_x0[inst[i].gx_off + s]);
+        tp->grad_finite[i] = ok ? 1 : 0;
+        n_bad += ok ? 0 : 1;
+    }
+    if (n_bad)
+        return fail(LASSIE_ERR_NONFINITE, std::to_string(n_bad) +
+                                              " gradient(s) not finite: a BDF1 step whose modified Newton iteration "
+                                              "did not converge (see lassie_adjoint_status)");
     return LASSIE_OK;
 #undef ADJ_TRY
 }

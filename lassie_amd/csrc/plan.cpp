@@ -212,7 +212,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     aw.done_host = P->h_done + slots;
 
     // autotuner: a short trial of every kernel variant on the plan's own
-    // workgroup-level instances (16 loop iterations, at most 256 Newton
+    // workgroup-level instances (8 loop iterations, at most 64 Newton
     // iterations each); the fastest is kept for this device and workload
     // shape, in this process and in the cache file (tune_cache.h)
     const auto key = std::make_tuple(P->device, smax, sp.n_wg, (int)o.precision);
@@ -239,8 +239,8 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         chosen = -1;
     if (chosen < 0) {
         lassie::BatchArgs trial = a;
-        trial.max_iters = 16;
-        trial.max_newton = 256;
+        trial.max_iters = 8;
+        trial.max_newton = 64;
         trial.done_host = nullptr;
         float best = 0.f;
         for (int v = 0; v < lassie::kBatchVariants; v++) {

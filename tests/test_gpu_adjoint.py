@@ -152,3 +152,33 @@ def test_differentiation_shape_512x1024(gpu_lib):
         print(f"  parameterisation {i}: steps={n} max newton={mx}", flush=True)
         assert np.abs(gks[i] - ogk).max() <= 1e-9 * np.abs(ogk).max(), i
     tape.close()
+
+
+def test_nonfinite_gradient_is_reported(gpu_lib):
+    """Parameterisations 0 and 108 of the 512 x 1024 workload on t in
+    [0, 0.2]: 108 has a BDF1 step whose modified Newton iteration ran all
+    10000 iterations without converging, so its gradient is not finite (the
+    oracle's exact adjoint overflows too).  The reverse pass returns
+    LASSIE_ERR_NONFINITE, lassie_adjoint_status names the model and counts the
+    capped Newton steps, and model 0's gradient equals the oracle's."""
+    ms, _ = synth.parameterisations(512, 1024, 109, seed=0)
+    ms = [ms[0], ms[108]]
+    t = synth.t_vector(0, 0.2, 50)
+    print("[adjoint] parameterisations 0 and 108 of 512x1024, t in [0, 0.2]", flush=True)
+    dms = [_lib.DeviceModel(m) for m in ms]
+    outs, _, _, tape = _lib.forward_batch(dms, t, **CTRL)
+    rng = np.random.default_rng(2)
+    Ws = [rng.standard_normal(o.shape) for o in outs]
+    assert tape.status(0)[0] == -1
+    with pytest.raises(_lib.LassieError) as err:
+        tape.adjoint(Ws)
+    assert err.value.code == _lib.LASSIE_ERR_NONFINITE
+    gks, gxs, _ = tape.adjoint(Ws, allow_nonfinite=True)
+    assert tape.status(0) == (1, 0)
+    finite, capped = tape.status(1)
+    assert finite == 0 and capped >= 1
+    assert np.isfinite(gks[0]).all() and not np.isfinite(gks[1]).all()
+    o_out, _, _, otape = O.solve_tape(ms[0], t, **CTRL)
+    ogk, _ = O.adjoint(ms[0], otape, Ws[0])
+    assert np.abs(gks[0] - ogk).max() <= 1e-9 * np.abs(ogk).max()
+    tape.close()

@@ -183,3 +183,31 @@ def test_plan_autotuner_choice_persists(gpu_lib, tmp_path):
     (key, entry), = doc["entries"].items()
     assert key.startswith("batch_variant/") and "/smax200/n6/prec0" in key
     assert entry["choice"] == v1 and len(entry["trial_ms"]) == 3
+
+
+def test_batch_fp64_bit_exact_large_c4_sizes(gpu_lib):
+    """Instances of the 2048-instance workload's generator at S = 128, 256
+    and 512 (R = 2S, k logU[1e-4, 1e4], t in [0, 50], 100 samples) on the
+    workgroup-level kernel against the oracle, bit for bit: S = 128 over the
+    whole horizon (with the bench's Newton cap), S = 256 and 512 over their
+    first 1500 loop iterations."""
+    models = synth.uneven_batch(6, 1000, sizes=(128, 256, 512))
+    t = synth.t_vector(0.0, 50.0, 100)
+    for cap in ((0, (128,)), (1500, (256, 512))):
+        ms = [m for m in models if m.S in cap[1]]
+        if not ms:
+            continue
+        kw = dict(BENCH, max_newton=50000, max_iters=cap[0])
+        print(f"[batch] large C4 sizes {[m.S for m in ms]} max_iters={cap[0]}", flush=True)
+        dms = [gpu_lib.DeviceModel(m) for m in ms]
+        outs, ys, stats, _, rc = gpu_lib.solve_batch(dms, t, raise_on_maxiter=False, **kw)
+        for dm in dms:
+            dm.close()
+        for m, out, y, st in zip(ms, outs, ys, stats):
+            o_out, o_st, o_rc, _, o_y = O.solve(m, t, **kw)
+            for key in ("loop_iters", "rkf_accepts", "bdf_steps", "newton_iters", "nan_repairs",
+                        "lu_failures", "samples"):
+                assert st[key] == o_st[key], (m.S, key, st[key], o_st[key])
+            n = int(o_st["samples"])
+            np.testing.assert_array_equal(out[:n], o_out[:n])
+            np.testing.assert_array_equal(y, o_y)
