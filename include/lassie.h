@@ -138,11 +138,29 @@ int lassie_solve(lassie_model *model, const lassie_opts *opts, const double *t, 
  * species); y_final (may be NULL) the S_i final states, model after model;
  * stats (may be NULL) one record per model; elapsed_ms (may be NULL) the
  * device time of the launch.  opts.precision selects the Newton linear solve
- * (LASSIE_PREC_*).  No counterpart in the reference, which runs one model per
- * process; each instance follows Simulation<T>::run (Simulation.cuh:632-925). */
+ * (LASSIE_PREC_*).  Rows of samples an instance did not reach (max_iters or
+ * max_newton) hold NaN.  No counterpart in the reference, which runs one
+ * model per process; each instance follows Simulation<T>::run
+ * (Simulation.cuh:632-925). */
 int lassie_solve_batch(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
                        const double *t, int32_t nt, double *out, double *y_final, lassie_stats *stats,
                        double *elapsed_ms);
+
+/* lassie_solve_batch without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments, results
+ * and output layout as lassie_solve_batch, for batches it can hold: FP64
+ * (opts.precision LASSIE_PREC_FP64), every model with at most 64 species and
+ * at most 2048 net-stoichiometry entries (the wave-level kernel), and the
+ * instance table, sampling times, counters and every output together within
+ * the library's fixed 64 MiB device buffer (one per GPU, held until the
+ * process ends; about 2 + 8 nt S bytes per model plus 1 KB).  Anything else
+ * returns LASSIE_ERR_UNSUPPORTED with the size it would need.  All calls on
+ * one GPU share that buffer: calls that overlap on one GPU (from several
+ * threads or streams) race, so callers serialise them. */
+int lassie_solve_batch_static(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
+                              const double *t, int32_t nt, double *out, double *y_final, lassie_stats *stats,
+                              double *elapsed_ms);
 
 /* Batched plan: the batched form prepared once and replayed (parameter sweeps,
  * repeated solves).  Creation allocates every workspace, picks the kernel

@@ -68,7 +68,7 @@ class Stats(C.Structure):
 
 # every symbol include/lassie.h declares
 EXPORTS = ("lassie_opts_default", "lassie_model_create", "lassie_model_destroy", "lassie_solve",
-           "lassie_solve_batch", "lassie_model_set_rates", "lassie_forward_batch",
+           "lassie_solve_batch", "lassie_solve_batch_static", "lassie_model_set_rates", "lassie_forward_batch",
            "lassie_adjoint_batch", "lassie_adjoint_status", "lassie_tape_info", "lassie_tape_read", "lassie_tape_destroy",
            "lassie_batch_plan_create", "lassie_batch_plan_run", "lassie_batch_plan_info",
            "lassie_batch_plan_progress", "lassie_batch_plan_timing", "lassie_batch_plan_destroy",
@@ -100,6 +100,7 @@ def lib():
         L.lassie_solve_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Opts), _f64p,
                                          C.c_int32, _f64p, _f64p, C.POINTER(Stats),
                                          C.POINTER(C.c_double)]
+        L.lassie_solve_batch_static.argtypes = L.lassie_solve_batch.argtypes
         L.lassie_csr_from_dense.argtypes = [C.c_int32, C.c_int32, _i16p, _i16p, _i32p, _i32p,
                                             _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]
         L.lassie_model_set_rates.argtypes = [C.c_void_p, _f64p]
@@ -227,9 +228,11 @@ class DeviceModel:
         return out, yf, d, rc, prog[:min(progress_cap, d["loop_iters"])]
 
 
-def solve_batch(models, t, raise_on_maxiter=True, **opts):
+def solve_batch(models, t, raise_on_maxiter=True, static=False, **opts):
     """Batched form over uploaded DeviceModels.  Returns (list of out[nt, S_i],
-    list of y_final[S_i], list of stats dicts, device ms, status)."""
+    list of y_final[S_i], list of stats dicts, device ms, status).  static:
+    the entry point that allocates no device memory
+    (lassie_solve_batch_static)."""
     t = np.ascontiguousarray(t, np.float64)
     n = len(models)
     sizes = [dm.S for dm in models]
@@ -239,8 +242,8 @@ def solve_batch(models, t, raise_on_maxiter=True, **opts):
     hs = (C.c_void_p * n)(*[dm.h for dm in models])
     ms = C.c_double(0.0)
     o = default_opts(**opts)
-    rc = lib().lassie_solve_batch(hs, n, C.byref(o), _p(t, _f64p), t.size, _p(out, _f64p),
-                                  _p(yf, _f64p), st, C.byref(ms))
+    fn = lib().lassie_solve_batch_static if static else lib().lassie_solve_batch
+    rc = fn(hs, n, C.byref(o), _p(t, _f64p), t.size, _p(out, _f64p), _p(yf, _f64p), st, C.byref(ms))
     if rc != LASSIE_OK and not (rc == LASSIE_ERR_MAXITER and not raise_on_maxiter):
         _check(rc)
     outs, ys = [], []

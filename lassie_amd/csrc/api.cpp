@@ -106,6 +106,13 @@ namespace lassie {
 int api_fail(int code, const std::string &msg) { return fail(code, msg); }
 int model_species(const lassie_model *m) { return m->S; }
 int model_device(const lassie_model *m) { return m->device; }
+int model_nnz_net(const lassie_model *m) { return m->hptr[m->S]; }
+hipStream_t model_stream(const lassie_model *m) { return m->stream; }
+void model_events(const lassie_model *m, hipEvent_t *ev0, hipEvent_t *ev1)
+{
+    *ev0 = m->ev0;
+    *ev1 = m->ev1;
+}
 
 void fill_batch_inst(const lassie_model *m, BatchInst *b)
 {
@@ -780,7 +787,7 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     BATCH_TRY(hipMemcpyAsync(d_inst, inst.data(), sizeof(lassie::BatchInst) * n_models, hipMemcpyHostToDevice, st));
     BATCH_TRY(hipMemcpyAsync(d_order, order.data(), sizeof(int) * n_models, hipMemcpyHostToDevice, st));
     BATCH_TRY(hipMemcpyAsync(d_tv, t, sizeof(double) * nt, hipMemcpyHostToDevice, st));
-    BATCH_TRY(hipMemsetAsync(d_next, 0, 2 * sizeof(int), st));
+    BATCH_TRY(lassie::launch_batch_reset(d_next, st, d_out, (size_t)out_tot));
     BATCH_TRY(hipMemsetAsync(d_wsi, 0, sizeof(int) * wsi_stride * slots, st));
     if (tape) BATCH_TRY(hipMemsetAsync(tape->d_sample, 0, sizeof(int) * (size_t)nt * n_models, st));
     lassie::BatchArgs a{};

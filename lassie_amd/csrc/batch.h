@@ -68,7 +68,9 @@ constexpr int kBatchDefaultVariant = 2;
 int batch_variant_rt(int variant);
 hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream, int variant = kBatchDefaultVariant);
 int batch_occupancy(int *per_cu, int variant = kBatchDefaultVariant);
-// Resets both work-queue heads (next[0], next[1]) inside a captured graph.
-hipError_t launch_batch_reset(int *next, hipStream_t stream);
+// Resets both work-queue heads (next[0], next[1]) and fills out[0, n_out)
+// with NaN (samples an instance does not reach keep that value); a kernel,
+// so it may be captured in a graph.
+hipError_t launch_batch_reset(int *next, hipStream_t stream, double *out = nullptr, size_t n_out = 0);
 
 }  // namespace lassie

@@ -402,15 +402,20 @@ hipError_t launch_batch(const BatchArgs &args, int slots, hipStream_t stream, in
     return hipGetLastError();
 }
 
-// Resets the work-queue heads inside a captured graph (a kernel node, not a memset node).
-__global__ void batch_reset_kernel(int *next)
+// Resets the two work-queue heads and marks every output sample as not
+// reached (NaN): a kernel, so that a captured graph holds no memset node.
+__global__ void batch_reset_kernel(int *next, double *out, size_t n_out)
 {
-    if (threadIdx.x < 2) next[threadIdx.x] = 0;
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 < 2) next[i0] = 0;
+    for (size_t i = i0; i < n_out; i += (size_t)gridDim.x * blockDim.x) out[i] = __builtin_nan("");
 }
 
-hipError_t launch_batch_reset(int *next, hipStream_t stream)
+hipError_t launch_batch_reset(int *next, hipStream_t stream, double *out, size_t n_out)
 {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, next);
+    size_t blocks = (n_out + NT - 1) / NT;
+    blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((unsigned)blocks), dim3(NT), 0, stream, next, out, n_out);
     return hipGetLastError();
 }
 

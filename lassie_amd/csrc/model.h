@@ -1,5 +1,7 @@
 // Library-internal helpers shared by the C ABI sources (api.cpp, plan.cpp).
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <string>
 #include <vector>
 
@@ -11,6 +13,9 @@ namespace lassie {
 int api_fail(int code, const std::string &msg);              // sets lassie_last_error, returns code
 int model_species(const lassie_model *m);
 int model_device(const lassie_model *m);
+int model_nnz_net(const lassie_model *m);                    // net-stoichiometry entries (RHS records)
+hipStream_t model_stream(const lassie_model *m);             // the model's stream
+void model_events(const lassie_model *m, hipEvent_t *ev0, hipEvent_t *ev1);
 void fill_batch_inst(const lassie_model *m, BatchInst *b);   // device pointers; offsets left to the caller
 void fill_batch_stats(const long long *c, const double *d, float ms, int slots, lassie_stats *s);
 

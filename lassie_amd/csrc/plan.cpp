@@ -278,7 +278,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
             return bail(LASSIE_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
         hipError_t ec = hipSuccess;
         if (g == 0) {
-            ec = lassie::launch_batch_reset(P->d_next, s);
+            ec = lassie::launch_batch_reset(P->d_next, s, P->d_out, (size_t)P->out_tot);
         } else if (g == 1) {
             if (sp.n_wg) ec = lassie::launch_batch(a, slots, s, P->variant);
         } else if (g == 2) {
