@@ -164,18 +164,21 @@ int lassie_solve_batch_static(lassie_model *const *models, int32_t n_models, con
 
 /* Batched plan: the batched form prepared once and replayed (parameter sweeps,
  * repeated solves).  Creation allocates every workspace, picks the kernel
- * variant (row tiles of the LU trailing update in flight) by timing a short
- * trial of each variant on these instances (cached per device, largest S,
- * count and precision), and captures the launch sequence as a hipGraph.
+ * variant (LASSIE_BATCH_VARIANTS of them: 1, 2 or 4 row tiles of the LU
+ * trailing update in flight) by timing a short trial of each variant on these
+ * instances (cached per device, largest S, count and precision), and captures
+ * the launch sequence as hipGraphs.
  * lassie_batch_plan_run replays it, with the output layout of
  * lassie_solve_batch; rates changed with lassie_model_set_rates are used by
  * the next run.  The models must outlive the plan.  lassie_batch_plan_info
- * reports the chosen variant, its row tiles and the trial times (ms, one per
- * variant; zeros when the choice came from the cache).  Choices persist in a
+ * reports the chosen variant, its row tiles and the trial times
+ * (LASSIE_BATCH_VARIANTS values in ms; zeros when the choice came from the
+ * cache).  Choices persist in a
  * JSON file ($LASSIE_TUNE_CACHE, else $XDG_CACHE_HOME/lassie/tune.json, else
  * ~/.cache/lassie/tune.json; LASSIE_TUNE_CACHE=off keeps them in the process),
  * keyed by device architecture, CU count, largest S, instance count and
  * precision.  No counterpart in the reference (one model per process). */
+#define LASSIE_BATCH_VARIANTS 3
 typedef struct lassie_batch_plan lassie_batch_plan;
 int lassie_batch_plan_create(lassie_model *const *models, int32_t n_models, const lassie_opts *opts,
                              const double *t, int32_t nt, lassie_batch_plan **plan);

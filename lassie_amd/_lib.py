@@ -29,6 +29,7 @@ PREC_FP64 = 0
 PREC_FP32_REFINE = 1
 PREC_BF16_REFINE = 2
 PREC_FP32_LOOP = 3
+BATCH_VARIANTS = 3
 
 _i32p = C.POINTER(C.c_int32)
 _i16p = C.POINTER(C.c_int16)
@@ -382,7 +383,7 @@ class BatchPlan:
         """(variant, row tiles in flight, trial ms per variant)."""
         v = np.zeros(1, np.int32)
         rt = np.zeros(1, np.int32)
-        ms = (C.c_float * 3)()
+        ms = (C.c_float * BATCH_VARIANTS)()
         _check(lib().lassie_batch_plan_info(self.h, _p(v, _i32p), _p(rt, _i32p), ms))
         return int(v[0]), int(rt[0]), list(ms)
 

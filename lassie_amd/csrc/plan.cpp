@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -24,6 +25,8 @@
 #include "model.h"
 #include "tune_cache.h"
 #include "wave.h"
+
+static_assert(lassie::kBatchVariants == LASSIE_BATCH_VARIANTS, "variant count of the C ABI");
 
 namespace {
 
@@ -56,6 +59,10 @@ struct lassie_batch_plan {
     hipGraph_t graph[4] = {};
     hipGraphExec_t exec[4] = {};
     float tune_ms[lassie::kBatchVariants] = {0};
+    int per_cu[lassie::kBatchVariants] = {0};  // resident workgroups per CU of each variant
+    int cus = 1;
+    // workgroups a variant launches: what it keeps resident, within the slots
+    int slots_for(int v) const { return std::max(1, std::min(slots, cus * per_cu[v])); }
     float last_ms[2] = {0.f, 0.f};  // device time of the last run's two kernels
     lassie::BatchArgs args{}, wargs{};
 };
@@ -125,13 +132,24 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     P->n_wave = sp.n_wave;
     P->smax = sp.smax_wg;
     P->wave_lds = sp.wave_lds;
-    int cus = 0, per_cu = 0, wave_per_cu = 0;
+    int cus = 0, per_cu_max = 0, wave_per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device) != hipSuccess)
         return bail(LASSIE_ERR_HIP, "device attribute query failed");
-    if (lassie::batch_occupancy(&per_cu) != 0 || per_cu < 1) return bail(LASSIE_ERR_HIP, "batch occupancy query failed");
+    for (int v = 0; v < lassie::kBatchVariants; v++) {
+        if (lassie::batch_occupancy(&P->per_cu[v], v) != 0 || P->per_cu[v] < 1)
+            return bail(LASSIE_ERR_HIP, "batch occupancy query failed");
+        per_cu_max = std::max(per_cu_max, P->per_cu[v]);
+    }
     if (sp.n_wave && (lassie::wave_occupancy(sp.wave_lds, &wave_per_cu) != 0 || wave_per_cu < 1))
         return bail(LASSIE_ERR_HIP, "wave kernel occupancy query failed");
-    P->slots = std::max(1, std::min(sp.n_wg, cus * per_cu));
+    // workspaces for the variant with the most resident workgroups; each
+    // variant launches as many as it can keep resident
+    P->slots = std::max(1, std::min(sp.n_wg, cus * per_cu_max));
+    // LASSIE_BATCH_SLOTS caps the workgroup-level kernel's resident
+    // workgroups (to leave CUs to other work on a shared GPU)
+    if (const char *cap = std::getenv("LASSIE_BATCH_SLOTS"))
+        if (std::atoi(cap) > 0) P->slots = std::min(P->slots, std::atoi(cap));
+    P->cus = cus;
     P->wave_blocks = std::max(1, std::min(sp.n_wave, cus * wave_per_cu));
     const int smax = P->smax, slots = P->slots;
     const int n_done = slots + P->wave_blocks;  // finished-instance counters: workgroup slots, then waves
@@ -234,6 +252,11 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         if (it != g_tuned.end()) chosen = it->second;
     }
     if (chosen < 0 && sp.n_wg == 0) chosen = lassie::kBatchDefaultVariant;
+    // LASSIE_BATCH_VARIANT forces a variant (no trials), for measurements
+    if (const char *fv = std::getenv("LASSIE_BATCH_VARIANT")) {
+        const int v = std::atoi(fv);
+        if (v >= 0 && v < lassie::kBatchVariants && *fv) chosen = v;
+    }
     if (chosen < 0 && !fkey.empty() && lassie::tune_lookup(fkey, &chosen) &&
         (chosen < 0 || chosen >= lassie::kBatchVariants))
         chosen = -1;
@@ -246,7 +269,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         for (int v = 0; v < lassie::kBatchVariants; v++) {
             if ((e = lassie::launch_batch_reset(P->d_next, P->stream)) != hipSuccess ||
                 (e = hipEventRecord(P->ev0, P->stream)) != hipSuccess ||
-                (e = lassie::launch_batch(trial, slots, P->stream, v)) != hipSuccess ||
+                (e = lassie::launch_batch(trial, P->slots_for(v), P->stream, v)) != hipSuccess ||
                 (e = hipEventRecord(P->ev1, P->stream)) != hipSuccess ||
                 (e = hipStreamSynchronize(P->stream)) != hipSuccess)
                 return bail(LASSIE_ERR_HIP, std::string("autotune trial: ") + hipGetErrorString(e));
@@ -280,7 +303,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         if (g == 0) {
             ec = lassie::launch_batch_reset(P->d_next, s, P->d_out, (size_t)P->out_tot);
         } else if (g == 1) {
-            if (sp.n_wg) ec = lassie::launch_batch(a, slots, s, P->variant);
+            if (sp.n_wg) ec = lassie::launch_batch(a, P->slots_for(P->variant), s, P->variant);
         } else if (g == 2) {
             if (sp.n_wave) ec = lassie::launch_wave(aw, P->wave_blocks, sp.wave_lds, s);
         } else {
@@ -331,7 +354,7 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
     bool maxit = false;
     for (int i = 0; i < P->n; i++) {
         const long long *c = P->h_st + (size_t)i * lassie::kBatchStats;
-        if (stats) lassie::fill_batch_stats(c, P->h_dst + 4 * (size_t)i, ms, P->slots, &stats[i]);
+        if (stats) lassie::fill_batch_stats(c, P->h_dst + 4 * (size_t)i, ms, P->slots_for(P->variant), &stats[i]);
         maxit |= c[9] == 1;
     }
     if (maxit)

@@ -126,7 +126,7 @@ def test_batch_plan_replays_the_batched_form(gpu_lib):
     ref, ref_y, ref_st, _, _ = _lib.solve_batch(dms, t, **kw)
     plan = _lib.BatchPlan(dms, t, **kw)
     variant, rt, trial = plan.info()
-    assert variant in (0, 1, 2) and rt in (1, 2, 4)
+    assert variant in range(_lib.BATCH_VARIANTS) and rt in (1, 2, 4)
     assert plan.progress() == 0
     for _ in range(2):
         outs, ys, st, ms, rc = plan.run()
@@ -177,12 +177,12 @@ def test_plan_autotuner_choice_persists(gpu_lib, tmp_path):
         assert r.returncode == 0, r.stderr
         runs.append(json.loads(r.stdout.strip().splitlines()[-1]))
     (v1, _, ms1), (v2, _, ms2) = runs
-    assert max(ms1) > 0 and ms2 == [0.0, 0.0, 0.0] and v1 == v2
+    assert max(ms1) > 0 and ms2 == [0.0] * _lib.BATCH_VARIANTS and v1 == v2
     doc = json.loads(cache.read_text())
     assert doc["format"] == "lassie-tune-1"
     (key, entry), = doc["entries"].items()
     assert key.startswith("batch_variant/") and "/smax200/n6/prec0" in key
-    assert entry["choice"] == v1 and len(entry["trial_ms"]) == 3
+    assert entry["choice"] == v1 and len(entry["trial_ms"]) == _lib.BATCH_VARIANTS
 
 
 def test_batch_fp64_bit_exact_large_c4_sizes(gpu_lib):
