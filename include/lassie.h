@@ -41,10 +41,16 @@ extern "C" {
 #define LASSIE_ERR_NONFINITE 8 /* a computed result is not finite; the outputs are written (see
                                 * lassie_adjoint_status) */
 
-/* Arithmetic of the Newton linear solve. */
+/* Arithmetic of the Newton linear solve.  The refined forms are
+ * experimental: each Newton correction meets a componentwise backward-error
+ * target of refine_tol, but trajectories are not within 1e-6 of the FP64
+ * method's (the loop's step decisions are discontinuous; per-instance median
+ * relative errors of about 1e-3 to 5e-2 on the stiff batched workloads).
+ * BF16 is emulated: the trailing-update operands are rounded to BF16 and
+ * multiplied on the FP32 matrix cores, with FP32 accumulation. */
 #define LASSIE_PREC_FP64 0          /* plain method: FP64 LU (cuSOLVER getrf/getrs seat) */
-#define LASSIE_PREC_FP32_REFINE 1   /* FP32 LU + FP64 iterative refinement */
-#define LASSIE_PREC_BF16_REFINE 2   /* BF16 LU + FP64 iterative refinement */
+#define LASSIE_PREC_FP32_REFINE 1   /* FP32 LU + FP64 iterative refinement (experimental) */
+#define LASSIE_PREC_BF16_REFINE 2   /* BF16-rounded LU + FP64 iterative refinement (experimental, batched only) */
 /* the whole loop in single precision: `-p float`, Simulation<float>
  * (simulator.cu:259-304); lassie_solve only */
 #define LASSIE_PREC_FP32_LOOP 3
