@@ -206,7 +206,7 @@ struct AdjWork {
 // corrections in X, factors in W.M; the repaired final iterate in y_out.
 // Returns false when the LU fails.
 __device__ bool bdf_iterates(const DevModel &m, const AdjInst &I, const double *y, double h, int iters, double *y_out,
-                             AdjWork &W, double *s_z, Shared &sh)
+                             AdjWork &W, double *s_z, int *s_pf, Shared &sh)
 {
     const int S = m.S;
     const int tid = threadIdx.x;
@@ -217,6 +217,7 @@ __device__ bool bdf_iterates(const DevModel &m, const AdjInst &I, const double *
     for (int r = tid; r < S; r += NT) jac_row<true>(m, y, W.M, S, r, h);
     __syncthreads();
     if (wg_lu_rm<double, false>(W.M, S, W.piv, W.perm_src, W.moved_to, W.rows, W.rowcnt, W.info, sh) != 0) return false;
+    compose_rows<kBatchNB>(W.perm_src, W.moved_to, S, s_pf);
     for (int j = 0; j < iters; j++) {
         const double *vj = W.V + (size_t)j * S;
         double *vn = W.V + (size_t)(j + 1) * S;
@@ -226,7 +227,7 @@ __device__ bool bdf_iterates(const DevModel &m, const AdjInst &I, const double *
             s_z[s] = -(vj[s] - y[s] - h * f);
         }
         __syncthreads();
-        lu_solve_wg<kBatchNB, double, true>(W.M, S, W.perm_src, W.moved_to, s_z);
+        lu_solve_rows(W.M, S, s_z, s_pf);
         for (int s = tid; s < S; s += NT) {
             xj[s] = s_z[s];
             vn[s] = (m.feed[s] == 0) ? vj[s] + s_z[s] : m.feed[s];
@@ -264,6 +265,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
 {
     __shared__ Shared sh;
     __shared__ double s_z[kBatchMaxS];
+    __shared__ int s_pf[kBatchMaxS];  // row order of the current factors (compose_rows)
     __shared__ int s_idx;
     const int tid = threadIdx.x;
     const int slot = blockIdx.x;
@@ -362,7 +364,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                 } else if (kind == -2) {
                     for (int s = tid; s < S; s += NT) yn[s] = y[s];
                     __syncthreads();
-                } else if (!bdf_iterates(m, I, y, dt, kind, yn, W, s_z, sh)) {
+                } else if (!bdf_iterates(m, I, y, dt, kind, yn, W, s_z, s_pf, sh)) {
                     status = 1;
                     break;
                 }
@@ -427,7 +429,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                     for (int s = tid; s < S; s += NT) W.lam[s] = W.yb[s];
                     __syncthreads();
                 } else {
-                    if (!bdf_iterates(m, I, y, dt, kind, W.w1, W, s_z, sh)) {
+                    if (!bdf_iterates(m, I, y, dt, kind, W.w1, W, s_z, s_pf, sh)) {
                         status = 1;
                         break;
                     }

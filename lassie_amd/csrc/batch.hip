@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
     // the instance's state vectors (y1, two stage vectors, k1..k5, w1, yv, rhs):
     // LDS, so the RKF45 stages and Newton updates make no global round trips
     __shared__ double s_vec[11 * kBatchMaxS];
-    static_assert(kBatchMaxS <= 2 * NT, "lu_solve_wg gathers at most two rows per thread");
+    static_assert(kBatchMaxS <= 2 * NT, "lu_solve_wg and lu_solve_rows hold at most two rows per thread");
     __shared__ int s_idx;
     const int tid = threadIdx.x;
     const int slot = blockIdx.x;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                     if (fp64) {
                         for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
                         __syncthreads();
-                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x, s_pf);
+                        lu_solve_rows(M, S, s_x, s_pf);
                     } else if (!refine_solve(M, M32, S, perm_src, moved_to, s_pf, m.atol, rhs, s_x, s_b32, A.refine_tol,
                                              A.refine_max_iter, st_ref, sh)) {
                         // refinement did not converge: FP64 LU for the rest of this step
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                         compose_rows<kBatchNB>(perm_src, moved_to, S, s_pf);
                         for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
                         __syncthreads();
-                        lu_solve_wg<kBatchNB, double, true>(M, S, perm_src, moved_to, s_x, s_pf);
+                        lu_solve_rows(M, S, s_x, s_pf);
                     }
                     double part = 0.0;
                     for (int s = tid; s < S; s += NT) {

@@ -59,6 +59,58 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lq, int i) { return 4 * lq + i; }
 };
 
+// Raw buffer access to one matrix: a load whose byte offset is kBufOob (at
+// or past the buffer's size) returns zero and a store to it is dropped, so
+// masked lanes need no branches, and the memory counters stay exact.  The
+// range check covers the per-lane offset only (the scalar offset is added
+// unchecked), so every out-of-range access puts kBufOob in the lane offset.
+constexpr int kBufOob = 0x7ffffff0;
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+// A pointer every lane of the wave holds, moved to scalar registers.  Inside
+// a called function arguments arrive in vector registers, and a buffer
+// resource or scalar offset the compiler cannot prove uniform is wrapped in a
+// loop over the distinct lane values; these make the uniformity explicit.
+template <typename P>
+__device__ __forceinline__ P *uniform_ptr(P *p)
+{
+    const uintptr_t u = reinterpret_cast<uintptr_t>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return reinterpret_cast<P *>(((uintptr_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(uniform_ptr(p)), (short)0, uniform_int(bytes),
+                                             0x00020000);
+}
+template <typename T>
+struct BufIO;
+template <>
+struct BufIO<double> {
+    static __device__ __forceinline__ double ld(__amdgpu_buffer_rsrc_t rs, int off, int soff = 0)
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, soff, 0));
+    }
+    // v is a named double: __builtin_bit_cast of a vector element (d[i]) reads
+    // element 0 in this compiler, so callers pass elements through a value
+    static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t rs, int off, double v, int soff = 0)
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), rs, off, soff, 0);
+    }
+};
+template <>
+struct BufIO<float> {
+    static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t rs, int off, int soff = 0)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, soff, 0));
+    }
+    static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t rs, int off, float v, int soff = 0)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), rs, off, soff, 0);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // workgroup reductions (all threads receive the result)
 
@@ -825,6 +877,190 @@ __device__ void lu_solve_wg(const T *__restrict__ M, int n, const int *perm_src,
     }
 }
 
+
+// getrs with one right-hand side for FP64 factors in row-major final row
+// order (pf of compose_rows), n <= 2 NT, b in LDS: the same fma chains as
+// lu_solve_wg (each row's forward chain k ascending from b, its backward
+// chain k descending, then the division by the diagonal), so the results are
+// bit-identical; organised for the memory system instead of the panels.
+// Thread t owns rows t and t + NT for the whole solve and keeps their
+// partial sums in registers.  Columns go in blocks of TB: the wave holding
+// the block's diagonal rows solves them with register shuffles and publishes
+// x through LDS, one barrier, then every thread applies the block to its
+// rows.  Each thread's TB-column row segments come from raw buffer loads
+// (out-of-range offsets read zeros, so unused segments cost no branches) into
+// two register sets: the set for block c + 2 is requested as soon as block c
+// is applied, so a block's loads have one block of work to arrive.
+// VEC: doubles per load (2 needs 16-byte aligned rows: n even, M aligned).
+template <int TB, int VEC>
+__device__ void lu_solve_rows_impl(const double *__restrict__ M, int n, double *b, const int *pf)
+{
+    M = uniform_ptr(M);
+    n = uniform_int(n);
+    constexpr int kOob = kBufOob;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+    double acc[2];
+    acc[0] = (tid < n) ? b[pf[tid]] : 0.0;
+    acc[1] = (tid + NT < n) ? b[pf[tid + NT]] : 0.0;
+    __syncthreads();
+    const int nc = (n + TB - 1) / TB;
+    struct Set {
+        double v[2][TB];
+    };
+    auto ld = [&](double *dst, int r, int k, bool use) {
+        const int off = use ? (r * n + k) * 8 : kOob;
+        if (VEC == 2) {
+            const double2 d = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+            dst[0] = d.x;
+            dst[1] = d.y;
+        } else {
+            dst[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+        }
+    };
+    // forward: row r needs L[r][k] for k < r
+    auto loadF = [&](Set &X, int c) {
+        const int k0 = c * TB;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int r = tid + q * NT;
+            const bool row_ok = c < nc && r < n;
+#pragma unroll
+            for (int j = 0; j < TB; j += VEC) ld(&X.v[q][j], r, k0 + j, row_ok && k0 + j < r);
+        }
+    };
+    // backward: row r needs U[r][k] for r <= k < n
+    auto loadB = [&](Set &X, int c) {
+        const int k0 = c * TB;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int r = tid + q * NT;
+            const bool row_ok = c >= 0 && r < n;
+#pragma unroll
+            for (int j = 0; j < TB; j += VEC)
+                ld(&X.v[q][j], r, k0 + j, row_ok && k0 + j + VEC - 1 >= r && k0 + j < n);
+        }
+    };
+    auto stepF = [&](Set &X, int c) {
+        const int k0 = c * TB, kb = min(TB, n - k0);
+        const int q = k0 / NT, dw = (k0 % NT) / 64, l0 = k0 % 64;
+        if (wave == dw) {
+            auto chain = [&](double &a, const double(&L)[TB]) {
+#pragma unroll
+                for (int kk = 0; kk < TB; kk++) {
+                    if (kb == TB || kk < kb) {
+                        const double xk = readlane_d(a, l0 + kk);
+                        if (lane > l0 + kk && lane < l0 + kb) a = fma(-L[kk], xk, a);
+                    }
+                }
+                if (lane >= l0 && lane < l0 + kb) b[k0 + lane - l0] = a;
+            };
+            if (q == 0)
+                chain(acc[0], X.v[0]);
+            else
+                chain(acc[1], X.v[1]);
+        }
+        __syncthreads();
+        double xs[TB];
+#pragma unroll
+        for (int kk = 0; kk < TB; kk++) xs[kk] = (kk < kb) ? b[k0 + kk] : 0.0;
+#pragma unroll
+        for (int q2 = 0; q2 < 2; q2++) {
+            const int r = tid + q2 * NT;
+            if (r < n && r >= k0 + kb) {
+#pragma unroll
+                for (int kk = 0; kk < TB; kk++)
+                    if (kb == TB || kk < kb) acc[q2] = fma(-X.v[q2][kk], xs[kk], acc[q2]);
+            }
+        }
+    };
+    auto stepB = [&](Set &X, int c) {
+        const int k0 = c * TB, kb = min(TB, n - k0);
+        const int q = k0 / NT, dw = (k0 % NT) / 64, l0 = k0 % 64;
+        if (wave == dw) {
+            auto chain = [&](double &a, const double(&U)[TB]) {
+#pragma unroll
+                for (int kk = TB - 1; kk >= 0; kk--) {
+                    if (kb == TB || kk < kb) {
+                        if (lane == l0 + kk) a = a / U[kk];
+                        const double xk = readlane_d(a, l0 + kk);
+                        if (lane >= l0 && lane < l0 + kk) a = fma(-U[kk], xk, a);
+                    }
+                }
+                if (lane >= l0 && lane < l0 + kb) b[k0 + lane - l0] = a;
+            };
+            if (q == 0)
+                chain(acc[0], X.v[0]);
+            else
+                chain(acc[1], X.v[1]);
+        }
+        __syncthreads();
+        double xs[TB];
+#pragma unroll
+        for (int kk = 0; kk < TB; kk++) xs[kk] = (kk < kb) ? b[k0 + kk] : 0.0;
+#pragma unroll
+        for (int q2 = 0; q2 < 2; q2++) {
+            const int r = tid + q2 * NT;
+            if (r < k0) {
+#pragma unroll
+                for (int kk = TB - 1; kk >= 0; kk--)
+                    if (kb == TB || kk < kb) acc[q2] = fma(-X.v[q2][kk], xs[kk], acc[q2]);
+            }
+        }
+    };
+    // the scheduling barriers keep each set's loads together and in program
+    // order, so the wait for set c leaves set c + 1 in flight
+#define LASSIE_SB() __builtin_amdgcn_sched_barrier(0)
+    {
+        Set X0, X1;
+        loadF(X0, 0);
+        LASSIE_SB();
+        loadF(X1, 1);
+        LASSIE_SB();
+        for (int c = 0; c < nc; c += 2) {
+            stepF(X0, c);
+            LASSIE_SB();
+            loadF(X0, c + 2);
+            LASSIE_SB();
+            if (c + 1 < nc) {
+                stepF(X1, c + 1);
+                LASSIE_SB();
+                loadF(X1, c + 3);
+                LASSIE_SB();
+            }
+        }
+    }
+    __syncthreads();
+    {
+        Set X0, X1;
+        loadB(X0, nc - 1);
+        LASSIE_SB();
+        loadB(X1, nc - 2);
+        LASSIE_SB();
+        for (int c = nc - 1; c >= 0; c -= 2) {
+            stepB(X0, c);
+            LASSIE_SB();
+            loadB(X0, c - 2);
+            LASSIE_SB();
+            if (c - 1 >= 0) {
+                stepB(X1, c - 1);
+                LASSIE_SB();
+                loadB(X1, c - 3);
+                LASSIE_SB();
+            }
+        }
+    }
+#undef LASSIE_SB
+    __syncthreads();
+}
+
+__device__ __forceinline__ void lu_solve_rows(const double *__restrict__ M, int n, double *b, const int *pf)
+{
+    if ((n & 1) == 0 && (reinterpret_cast<uintptr_t>(M) & 15) == 0)
+        lu_solve_rows_impl<16, 2>(M, n, b, pf);
+    else
+        lu_solve_rows_impl<16, 1>(M, n, b, pf);
+}
 
 __device__ __forceinline__ bool bad_value(double v) { return isnan(v) || isinf(v) || v < 0; }
 

@@ -1,6 +1,7 @@
 // Workgroup-local dense LU of the batched and differentiation forms: one
 // workgroup factors one instance's row-major Newton matrix (S <= 512).
 #pragma once
+#include <type_traits>
 #include "batch.h"
 #include "device_common.h"
 #include "wg_common.h"
@@ -27,10 +28,100 @@ constexpr int kLpStride = 17;  // LDS row stride of the staged L21 panel (elemen
 // every precision's instantiation of wg_lu_rm in a kernel
 static __shared__ double lp_raw[(kBatchMaxS - kBatchNB) * kLpStride];
 
+// Trailing update A22 -= L21 U12 of an FP64 factorization with n % 16 == 0
+// (every tile full, so no lane checks): the same four 16x16x4 MFMAs per
+// output tile as the general form (k ascending, the FP64 instruction's
+// sequential fma chain), with the per-lane address parts fixed for the whole
+// update and the per-tile parts in scalar offsets.  A wave walks its tiles
+// (column tiles ct = wave (mod 4), every row tile of each) through a ring of
+// D sets, each holding one tile's output block and U12 operands in flight;
+// the L21 operands come from the LDS panel (Lp, kLpStride per row).
+template <int D, typename LT>
+__device__ void trailing_update_full(double *M, int n, int r0, int c0, const LT *Lp)
+{
+    M = uniform_ptr(M);
+    n = uniform_int(n);
+    r0 = uniform_int(r0);
+    c0 = uniform_int(c0);
+    typedef typename Mfma<double>::acc Acc;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+    const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, lq = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nt = (n - c0) >> 4;
+    const int W = wave < nt ? ((nt - wave + NWAVES - 1) / NWAVES) * nt : 0;
+    // output element (c0 + lq + 4i + 16g, c0 + l16 + 16ct) at vc + 32n i + 128n g + 128 ct bytes
+    const int vc = ((c0 + lq) * n + c0 + l16) * 8;
+    // U12 element (r0 + lq + 4kc, c0 + l16 + 16ct) at vb + 32n kc + 128 ct
+    const int vb = ((r0 + lq) * n + c0 + l16) * 8;
+    // L21 element (16g + l16, lq + 4kc) of the LDS panel
+    const LT *la = Lp + l16 * kLpStride + lq;
+    struct Set {
+        Acc d;
+        double b[4];
+    };
+    auto issue = [&](Set &st, int g, int ct) {
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) st.b[kc] = BufIO<double>::ld(rs, vb, 128 * ct + 32 * n * kc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) st.d[i] = BufIO<double>::ld(rs, vc, 128 * n * g + 128 * ct + 32 * n * i);
+    };
+    auto finish = [&](Set &st, int g, int ct) {
+        const LT *ap = la + 16 * kLpStride * g;
+        double a[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) a[kc] = (double)ap[4 * kc];
+        Acc d = st.d;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) d = Mfma<double>::run(a[kc], st.b[kc], d);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double v = d[i];
+            BufIO<double>::st(rs, vc, v, 128 * n * g + 128 * ct + 32 * n * i);
+        }
+    };
+    // tile w of the wave: column tile ct = wave + 4 (w / nt), row tile g = w % nt
+    int gi = 0, ci = wave, gf = 0, cf = wave;
+    auto next = [&](int &g, int &ct) {
+        if (++g == nt) {
+            g = 0;
+            ct += NWAVES;
+        }
+    };
+    // the scheduling barriers keep each set's memory operations in program
+    // order, so the wait for one set leaves the later sets in flight
+    Set S[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        if (j < W) {
+            issue(S[j], gi, ci);
+            next(gi, ci);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int w = 0; w < W; w += D) {
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            if (w + j < W) {
+                finish(S[j], gf, cf);
+                next(gf, cf);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (w + j + D < W) {
+                issue(S[j], gi, ci);
+                next(gi, ci);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
 template <typename T, bool BF16, int RT = 4>
 __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
                         int *info, Shared &sh)
 {
+    // uniform across the workgroup: scalar registers, exact buffer accesses
+    M = uniform_ptr(M);
+    n = uniform_int(n);
     constexpr int NBC = kBatchNB;
     constexpr int RPT = kBatchMaxS / NT;
     const int P = (n + NBC - 1) / NBC;
@@ -58,34 +149,54 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
             if (kk < nbp && c < kk) sh.L11[kk][c] = (double)M[(size_t)(r0 + kk) * n + r0 + c];
         }
         __syncthreads();
-        // row moves in every other column; U block row by forward substitution
-        for (int c = tid; c < n; c += NT) {
-            if (c >= r0 && c < r0 + nbp) continue;
-            T vals[NBC], old[NBC];
+        // row moves in every other column; U block row by forward substitution.
+        // A thread takes columns tid and tid + NT together and issues every
+        // load of both before the first use (one memory round trip); masked
+        // lanes use out-of-range buffer offsets instead of branches.
+        {
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * (int)sizeof(T));
+            int src[NBC], dst[NBC];
 #pragma unroll
             for (int kk = 0; kk < NBC; kk++) {
-                vals[kk] = (kk < nbp) ? M[(size_t)perm_src[r0 + kk] * n + c] : T(0);
-                old[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + c] : T(0);
+                src[kk] = (kk < nbp) ? perm_src[r0 + kk] : 0;
+                dst[kk] = (kk < nbp) ? moved_to[r0 + kk] : -1;
             }
+            T vals[2][NBC], old[2][NBC];
 #pragma unroll
-            for (int kk = 0; kk < NBC; kk++) {
-                if (kk < nbp) {
-                    const int d = moved_to[r0 + kk];
-                    if (d >= r0 + nbp) M[(size_t)d * n + c] = old[kk];
+            for (int j = 0; j < 2; j++) {
+                const int c = tid + j * NT;
+                const bool okc = c < n && (c < r0 || c >= r0 + nbp);
+#pragma unroll
+                for (int kk = 0; kk < NBC; kk++) {
+                    const bool ok = okc && kk < nbp;
+                    vals[j][kk] = BufIO<T>::ld(rs, ok ? (src[kk] * n + c) * (int)sizeof(T) : kBufOob);
+                    old[j][kk] = BufIO<T>::ld(rs, ok ? ((r0 + kk) * n + c) * (int)sizeof(T) : kBufOob);
                 }
             }
-            if (c >= r0 + nbp) {
 #pragma unroll
-                for (int i = 1; i < NBC; i++) {
-                    if (i < nbp) {
+            for (int j = 0; j < 2; j++) {
+                const int c = tid + j * NT;
+                const bool okc = c < n && (c < r0 || c >= r0 + nbp);
 #pragma unroll
-                        for (int k = 0; k < i; k++) vals[i] = fma(-(T)sh.L11[i][k], vals[k], vals[i]);
+                for (int kk = 0; kk < NBC; kk++) {
+                    const bool ok = okc && kk < nbp && dst[kk] >= r0 + nbp;
+                    BufIO<T>::st(rs, ok ? (dst[kk] * n + c) * (int)sizeof(T) : kBufOob, old[j][kk]);
+                }
+                if (okc && c >= r0 + nbp) {
+#pragma unroll
+                    for (int i = 1; i < NBC; i++) {
+                        if (i < nbp) {
+#pragma unroll
+                            for (int k = 0; k < i; k++) vals[j][i] = fma(-(T)sh.L11[i][k], vals[j][k], vals[j][i]);
+                        }
                     }
                 }
-            }
 #pragma unroll
-            for (int kk = 0; kk < NBC; kk++)
-                if (kk < nbp) M[(size_t)(r0 + kk) * n + c] = vals[kk];
+                for (int kk = 0; kk < NBC; kk++) {
+                    const bool ok = okc && kk < nbp;
+                    BufIO<T>::st(rs, ok ? ((r0 + kk) * n + c) * (int)sizeof(T) : kBufOob, vals[j][kk]);
+                }
+            }
         }
         __syncthreads();
         // trailing update A22 -= L21 U12 on the matrix cores: 16 x 16 output
@@ -102,13 +213,35 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
         if (c0 < n) {
             typedef __attribute__((address_space(3))) T LT;
             LT *Lp = (LT *)lp_raw;
-            for (int e = tid; e < (n - c0) * NBC; e += NT) {
-                const int rr = e / NBC, k = e % NBC;
-                T v = (k < nbp) ? -M[(size_t)(c0 + rr) * n + r0 + k] : T(-0.0);
-                if (BF16) v = (T)to_bf16_rne((float)v);
-                Lp[rr * kLpStride + k] = v;
+            {
+                // every element's load in flight at once, then the LDS stores
+                constexpr int kStage = ((kBatchMaxS - NBC) * NBC + NT - 1) / NT;
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * (int)sizeof(T));
+                const int cnt = (n - c0) * NBC;
+                T v[kStage];
+#pragma unroll
+                for (int j = 0; j < kStage; j++) {
+                    const int e = tid + j * NT, rr = e / NBC, k = e % NBC;
+                    const bool ok = e < cnt && k < nbp;
+                    v[j] = BufIO<T>::ld(rs, ok ? ((c0 + rr) * n + r0 + k) * (int)sizeof(T) : kBufOob);
+                }
+#pragma unroll
+                for (int j = 0; j < kStage; j++) {
+                    const int e = tid + j * NT, rr = e / NBC, k = e % NBC;
+                    if (e < cnt) {
+                        T x = (k < nbp) ? -v[j] : T(-0.0);
+                        if (BF16) x = (T)to_bf16_rne((float)x);
+                        Lp[rr * kLpStride + k] = x;
+                    }
+                }
             }
             __syncthreads();
+            if constexpr (std::is_same<T, double>::value && !BF16) {
+                if ((n & 15) == 0) {
+                    trailing_update_full<2 * RT>(M, n, r0, c0, Lp);
+                    goto update_done;
+                }
+            }
             typedef typename Mfma<T>::acc Acc;
             // global-memory view of M (the workspace lives in device memory):
             // global loads/stores keep the memory counters precise, so the
@@ -187,6 +320,7 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
                 }
             }
         }
+    update_done:
         __syncthreads();
     }
     return 0;

@@ -221,3 +221,33 @@ def test_batch_fp64_bit_exact_large_c4_sizes(gpu_lib):
             n = int(o_st["samples"])
             np.testing.assert_array_equal(out[:n], o_out[:n])
             np.testing.assert_array_equal(y, o_y)
+
+
+def test_batch_fp64_bit_exact_odd_sizes(gpu_lib):
+    """Odd and even S above the wave-level limit in one batch whose largest
+    instance is odd (S in {66, 97, 130, 131}): the per-slot workspace stride
+    is then odd, so factors sit at 8-byte as well as 16-byte aligned
+    addresses, and the Newton solve runs with both of its load widths (8-byte
+    loads for odd S or unaligned factors, 16-byte loads otherwise).  Every
+    instance against the oracle, bit for bit, over its first 3000 loop
+    iterations (k logU[1e-4, 1e4], t in [0, 20], 40 samples)."""
+    models = synth.uneven_batch(8, 7100, sizes=(66, 97, 130, 131))
+    models[0] = synth.stiff(131, 262, 7100, (1e-4, 1e4))
+    t = synth.t_vector(0.0, 20.0, 40)
+    kw = dict(BENCH, max_newton=50000, max_iters=3000)
+    print(f"[batch] odd sizes {[m.S for m in models]}", flush=True)
+    dms = [gpu_lib.DeviceModel(m) for m in models]
+    outs, ys, stats, _, rc = gpu_lib.solve_batch(dms, t, raise_on_maxiter=False, **kw)
+    for dm in dms:
+        dm.close()
+    newton = 0
+    for m, out, y, st in zip(models, outs, ys, stats):
+        o_out, o_st, o_rc, _, o_y = O.solve(m, t, **kw)
+        for key in ("loop_iters", "rkf_accepts", "bdf_steps", "newton_iters", "nan_repairs",
+                    "lu_failures", "samples"):
+            assert st[key] == o_st[key], (m.S, key, st[key], o_st[key])
+        n = int(o_st["samples"])
+        np.testing.assert_array_equal(out[:n], o_out[:n])
+        np.testing.assert_array_equal(y, o_y)
+        newton += st["newton_iters"]
+    assert newton > 0  # the Newton solve ran
