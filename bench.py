@@ -176,11 +176,12 @@ def main():
     # work, so further warmups change nothing and the budget goes to timed
     # passes instead.
     plain_reserve = 45.0 * args.plain_per_size + 15.0
-    warm = 0
+    warm, first_pass = 0, None
     if args.warmup > 0:
-        say("warmup pass")
-        run_once()
+        say("warmup pass (instances handed out largest S first)")
+        _, first_pass = run_once()
         warm = 1
+        say(f"  warmup pass: {first_pass:.2f} s; later passes hand instances out longest measured time first")
     say("timed passes")
     walls, kernel_ms, res = [], [], None
     while len(walls) < args.steps:
@@ -263,6 +264,10 @@ def main():
                      "wave_kernel": {"instances": n_wave, "ms": wv_ms, "flops": wv_flops,
                                      "tflops": wv_flops / (wv_ms / 1e3) / 1e12 if wv_ms > 0 else None}},
         "per_size": per_size, "device_ms_per_step": dev_ms, "walls_s": walls,
+        "schedule": {"first_pass_s": first_pass,
+                     "note": "the plan's first run hands instances out largest S first (first_pass_s, the warmup "
+                             "pass); every run records each instance's device time and the next run hands them "
+                             "out longest first, so the timed passes run the measured-cost schedule"},
     }
     traffic = os.path.join(ROOT, "profiles", "traffic_batch_c4.json")
     if os.path.exists(traffic):

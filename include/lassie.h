@@ -176,7 +176,11 @@ int lassie_solve_batch_static(lassie_model *const *models, int32_t n_models, con
  * the launch sequence as hipGraphs.
  * lassie_batch_plan_run replays it, with the output layout of
  * lassie_solve_batch; rates changed with lassie_model_set_rates are used by
- * the next run.  The models must outlive the plan.  lassie_batch_plan_info
+ * the next run.  The first run hands instances to the kernels' work queues
+ * largest S first; each run records every instance's device time, and the
+ * next run hands them out longest first (list scheduling on the measured
+ * costs; LASSIE_BATCH_SCHEDULE=size keeps the by-size order).  The order
+ * changes no result, only when each instance runs.  The models must outlive the plan.  lassie_batch_plan_info
  * reports the chosen variant, its row tiles and the trial times
  * (LASSIE_BATCH_VARIANTS values in ms; zeros when the choice came from the
  * cache).  Choices persist in a

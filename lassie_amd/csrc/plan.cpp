@@ -5,7 +5,12 @@
 // LU trailing update in flight per wave) by timing a short trial of every
 // variant on the plan's own instances, and captures the launch sequence (a
 // queue-reset kernel, the persistent kernel, copies into the pinned buffers)
-// as a hipGraph.  lassie_batch_plan_run replays the graph.  Rates changed
+// as a hipGraph.  lassie_batch_plan_run replays the graph.  The first run
+// hands instances out largest S first; every run records each instance's
+// device time (the kernels' phase timers), and the next run hands them out
+// longest first (LPT list scheduling on the measured costs), so repeated
+// runs of one plan, e.g. a parameter sweep, finish their slowest instances
+// earlier.  Rates changed
 // with lassie_model_set_rates are used by the next run, since the graph reads
 // the models' device arrays.  Each instance follows Simulation<T>::run
 // (Simulation.cuh:632-925), as in lassie_solve_batch.
@@ -64,6 +69,11 @@ struct lassie_batch_plan {
     // workgroups a variant launches: what it keeps resident, within the slots
     int slots_for(int v) const { return std::max(1, std::min(slots, cus * per_cu[v])); }
     float last_ms[2] = {0.f, 0.f};  // device time of the last run's two kernels
+    // the work queues' order: workgroup-level instances [0, n_wg), then
+    // wave-level ones; by S for the first run, then by measured device time
+    std::vector<int> order;
+    bool measured_order = true;   // reorder by the last run's device times
+    int runs = 0;
     lassie::BatchArgs args{}, wargs{};
 };
 
@@ -128,6 +138,10 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     const lassie::BatchSplit sp =
         lassie::split_batch(models, inst, o.precision == LASSIE_PREC_FP64 && !(o.reserved & 2));
     const std::vector<int> &order = sp.order;
+    P->order = sp.order;
+    // LASSIE_BATCH_SCHEDULE=size keeps the by-size order on every run
+    if (const char *sch = std::getenv("LASSIE_BATCH_SCHEDULE"))
+        if (std::strcmp(sch, "size") == 0) P->measured_order = false;
     P->n_wg = sp.n_wg;
     P->n_wave = sp.n_wave;
     P->smax = sp.smax_wg;
@@ -356,6 +370,20 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
         const long long *c = P->h_st + (size_t)i * lassie::kBatchStats;
         if (stats) lassie::fill_batch_stats(c, P->h_dst + 4 * (size_t)i, ms, P->slots_for(P->variant), &stats[i]);
         maxit |= c[9] == 1;
+    }
+    P->runs++;
+    if (P->measured_order) {
+        // longest measured device time first within each kernel's queue (ties
+        // keep the previous order); the next run's graph reads d_order
+        auto cost = [&](int i) {
+            const long long *c = P->h_st + (size_t)i * lassie::kBatchStats;
+            return c[14] + c[15] + c[16] + c[17];
+        };
+        auto longest = [&](int a, int b) { return cost(a) > cost(b); };
+        std::stable_sort(P->order.begin(), P->order.begin() + P->n_wg, longest);
+        std::stable_sort(P->order.begin() + P->n_wg, P->order.end(), longest);
+        if ((e = hipMemcpy(P->d_order, P->order.data(), sizeof(int) * P->n, hipMemcpyHostToDevice)) != hipSuccess)
+            return api_fail(LASSIE_ERR_HIP, std::string("plan schedule: ") + hipGetErrorString(e));
     }
     if (maxit)
         return api_fail(LASSIE_ERR_MAXITER,

@@ -115,10 +115,426 @@ __device__ void trailing_update_full(double *M, int n, int r0, int c0, const LT 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-level blocked form of the FP64 factorization (n % 16 == 0): blocks of
+// kLuBlock columns, each factored as four 16-column panels whose row moves,
+// U blocks and trailing updates stay inside the block; then the block's row
+// moves are applied once to the other columns, its U12 rows are solved, and
+// the trailing matrix takes one rank-64 update (sixteen 16x16x4 MFMAs per
+// output tile).  Every element still receives its updates as one fma chain
+// fma(-l_ik, u_kj, a_ij) over k ascending, the pivots are chosen from the
+// same values, and row moves are data movement only, so the factors are
+// bit-identical to the 16-column form; the trailing matrix is read and
+// written once per 64 columns instead of once per 16.
+constexpr int kLuBlock = 64;
+constexpr int kLuBlockedMin = 256;  // the blocked form factors n > kLuBlockedMin
+
+// composed row order of the current block (position -> source position at
+// the block's start) and the positions it changes (at most 2 kLuBlock)
+static __shared__ int lu_pfb[kBatchMaxS];
+static __shared__ int lu_chg[2 * kLuBlock];
+static __shared__ int lu_nchg;
+
+// -M[r][k0 + k] for rows rs <= r < re (re - rs <= MAXR), k < 16, into
+// Lp[(r - rs) * kLpStride + k]: every load in flight at once, then the LDS
+// stores.
+template <int MAXR, typename LT>
+__device__ __forceinline__ void stage_l16(const double *M, int n, int k0, int rs, int re, LT *Lp)
+{
+    constexpr int kPer = (MAXR * kBatchNB + NT - 1) / NT;
+    const __amdgpu_buffer_rsrc_t rsc = buf_rsrc(M, n * n * 8);
+    const int tid = threadIdx.x;
+    const int cnt = (re - rs) * kBatchNB;
+    double v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int e = tid + j * NT, rr = e >> 4, k = e & 15;
+        v[j] = BufIO<double>::ld(rsc, e < cnt ? ((rs + rr) * n + k0 + k) * 8 : kBufOob);
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int e = tid + j * NT, rr = e >> 4, k = e & 15;
+        if (e < cnt) Lp[rr * kLpStride + k] = -v[j];
+    }
+}
+
+// Row moves of the panel at r0 (16 rows) and the U-block forward
+// substitution, in the columns c_lo <= c < c_hi outside the panel (at most
+// 2 NT of them; thread t takes c_lo + t and c_lo + t + NT): with MOVES the
+// panel's row moves in every such column, and in the columns right of the
+// panel the substitution with the panel's L11 (in sh.L11), k ascending.
+// Masked lanes use out-of-range buffer offsets instead of branches.
+template <bool MOVES>
+__device__ void panel_rows_update(double *M, int n, int r0, int c_lo, int c_hi, const int *perm_src,
+                                  const int *moved_to, Shared &sh)
+{
+    constexpr int NBC = kBatchNB;
+    const int tid = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+    int src[NBC], dst[NBC];
+#pragma unroll
+    for (int kk = 0; kk < NBC; kk++) {
+        src[kk] = MOVES ? perm_src[r0 + kk] : r0 + kk;
+        dst[kk] = MOVES ? moved_to[r0 + kk] : -1;
+    }
+    double vals[2][NBC], old[2][NBC];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int c = c_lo + tid + j * NT;
+        const bool okc = c < c_hi && (c < r0 || c >= r0 + NBC);
+#pragma unroll
+        for (int kk = 0; kk < NBC; kk++) {
+            vals[j][kk] = BufIO<double>::ld(rs, okc ? (src[kk] * n + c) * 8 : kBufOob);
+            if (MOVES) old[j][kk] = BufIO<double>::ld(rs, okc ? ((r0 + kk) * n + c) * 8 : kBufOob);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int c = c_lo + tid + j * NT;
+        const bool okc = c < c_hi && (c < r0 || c >= r0 + NBC);
+        if (MOVES) {
+#pragma unroll
+            for (int kk = 0; kk < NBC; kk++) {
+                const bool ok = okc && dst[kk] >= r0 + NBC;
+                BufIO<double>::st(rs, ok ? (dst[kk] * n + c) * 8 : kBufOob, old[j][kk]);
+            }
+        }
+        if (okc && c >= r0 + NBC) {
+#pragma unroll
+            for (int i = 1; i < NBC; i++) {
+#pragma unroll
+                for (int k = 0; k < i; k++) vals[j][i] = fma(-sh.L11[i][k], vals[j][k], vals[j][i]);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < NBC; kk++) BufIO<double>::st(rs, okc ? ((r0 + kk) * n + c) * 8 : kBufOob, vals[j][kk]);
+    }
+}
+
+// A -= L U over the rectangle of output tiles rows [rb, rb + 16 nrt) x
+// columns [cb, cb + 16 nct): U rows [r0, r0 + 16) of the same columns, the
+// negated L columns staged in Lp (row r at (r - rb) * kLpStride), four
+// 16x16x4 MFMAs per tile, k ascending.  The tiles are dealt to the waves in
+// turn; each wave keeps a ring of D tiles' loads in flight.
+template <int D, typename LT>
+__device__ void tile_update16(double *M, int n, int r0, int rb, int nrt, int cb, int nct, const LT *Lp)
+{
+    M = uniform_ptr(M);
+    n = uniform_int(n);
+    r0 = uniform_int(r0);
+    rb = uniform_int(rb);
+    nrt = uniform_int(nrt);
+    cb = uniform_int(cb);
+    nct = uniform_int(nct);
+    typedef Mfma<double>::acc Acc;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+    const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, lq = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int total = nrt * nct;
+    const int W = wave < total ? (total - wave + NWAVES - 1) / NWAVES : 0;
+    // output element (rb + 16g + lq + 4i, cb + 16ct + l16) at vc + 128n g + 128 ct + 32n i
+    const int vc = ((rb + lq) * n + cb + l16) * 8;
+    // U element (r0 + lq + 4kc, cb + 16ct + l16) at vb + 32n kc + 128 ct
+    const int vb = ((r0 + lq) * n + cb + l16) * 8;
+    const LT *la = Lp + l16 * kLpStride + lq;
+    struct Set {
+        Acc d;
+        double b[4];
+    };
+    auto tile = [&](int w, int &g, int &ct) {
+        const int idx = wave + NWAVES * w;
+        g = idx / nct;
+        ct = idx - g * nct;
+    };
+    auto issue = [&](Set &st, int w) {
+        int g, ct;
+        tile(w, g, ct);
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) st.b[kc] = BufIO<double>::ld(rs, vb, 128 * ct + 32 * n * kc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) st.d[i] = BufIO<double>::ld(rs, vc, 128 * n * g + 128 * ct + 32 * n * i);
+    };
+    auto finish = [&](Set &st, int w) {
+        int g, ct;
+        tile(w, g, ct);
+        const LT *ap = la + 16 * kLpStride * g;
+        double a[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) a[kc] = (double)ap[4 * kc];
+        Acc d = st.d;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) d = Mfma<double>::run(a[kc], st.b[kc], d);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double v = d[i];
+            BufIO<double>::st(rs, vc, v, 128 * n * g + 128 * ct + 32 * n * i);
+        }
+    };
+    Set S[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        if (j < W) issue(S[j], j);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int w = 0; w < W; w += D) {
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            if (w + j < W) finish(S[j], w + j);
+            __builtin_amdgcn_sched_barrier(0);
+            if (w + j + D < W) issue(S[j], w + j + D);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// Rank-64 trailing update of a block: A22 -= L21 U12 for rows and columns
+// [E, n), k = R0 .. R0 + 63 ascending (sixteen 16x16x4 MFMAs per output
+// tile, one fma chain per element).  The negated L21 goes through LDS (lds,
+// up to 128 rows x 64 at a time) in the MFMA's lane order, so each operand
+// read is one contiguous 512-byte row.  Each wave owns the column tiles
+// ct = wave (mod 4) and walks them in pairs: the U12 operands of the pair
+// sit in two register banks, and every row tile of the chunk has one slot
+// whose output tile is in flight, filled for the next column tile as soon
+// as the current one is stored.  Slots, banks and loads are all fixed at
+// compile time (masked lanes and tiles past the end use out-of-range buffer
+// offsets), so the memory counters stay exact and only the tile being
+// finished is waited for.
+__device__ void tile_update64(double *M, int n, int R0, int E, double *lds)
+{
+    M = uniform_ptr(M);
+    n = uniform_int(n);
+    R0 = uniform_int(R0);
+    E = uniform_int(E);
+    typedef Mfma<double>::acc Acc;
+    typedef __attribute__((address_space(3))) double LT;
+    LT *La = (LT *)lds;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+    const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, lq = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nt = (n - E) >> 4;
+    const int ncw = wave < nt ? (nt - wave + NWAVES - 1) / NWAVES : 0;  // column tiles of this wave
+    // output element (E + 16g + lq + 4i, E + 16ct + l16) at vd + 128n g + 128 ct + 32n i
+    const int vd = ((E + lq) * n + E + l16) * 8;
+    // U12 element (R0 + lq + 4kc, E + 16ct + l16) at vb + 32n kc + 128 ct
+    const int vb = ((R0 + lq) * n + E + l16) * 8;
+    constexpr int kChunk = 8;  // row tiles staged at a time (LDS: kChunk * 16 * 64 doubles)
+    const int nchunk = (nt + kChunk - 1) / kChunk;
+    const int per = nchunk ? (nt + nchunk - 1) / nchunk : 0;
+    for (int rc = 0; rc < nt; rc += per) {
+        const int ng = min(per, nt - rc);
+        __syncthreads();  // the previous chunk's operands are no longer read
+        {
+            // -L21 of row tile g, MFMA step kc, lane l at (16 g + kc) * 64 + l
+            constexpr int kPer = kChunk * 16 * kLuBlock / NT;
+            const int cnt = ng * 16 * kLuBlock;
+            double v[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; j++) {
+                const int e = tid + j * NT, rr = e >> 6, k = e & 63;
+                v[j] = BufIO<double>::ld(rs, e < cnt ? ((E + 16 * rc + rr) * n + R0 + k) * 8 : kBufOob);
+            }
+#pragma unroll
+            for (int j = 0; j < kPer; j++) {
+                const int e = tid + j * NT, rr = e >> 6, k = e & 63;
+                if (e < cnt) La[((rr >> 4) * 16 + (k >> 2)) * 64 + (rr & 15) + 16 * (k & 3)] = -v[j];
+            }
+        }
+        __syncthreads();
+        if (ncw == 0) continue;
+        auto loadb = [&](double(&b)[16], int jc) {
+            const int off = jc < ncw ? vb : kBufOob;
+            const int ct = wave + NWAVES * jc;
+#pragma unroll
+            for (int kc = 0; kc < 16; kc++) b[kc] = BufIO<double>::ld(rs, off, 128 * ct + 32 * n * kc);
+        };
+        auto issue = [&](Acc &d, int jc, int g) {
+            const int off = (jc < ncw && g < ng) ? vd : kBufOob;
+            const int ct = wave + NWAVES * jc;
+#pragma unroll
+            for (int i = 0; i < 4; i++) d[i] = BufIO<double>::ld(rs, off, 128 * n * (rc + g) + 128 * ct + 32 * n * i);
+        };
+        auto finish = [&](const Acc &dset, int jc, int g, const double(&b)[16]) {
+            const bool live = jc < ncw && g < ng;
+            const int off = live ? vd : kBufOob;
+            const int ct = wave + NWAVES * jc;
+            Acc d = dset;
+            if (live) {
+                const LT *ap = La + g * 16 * 64 + lane;
+#pragma unroll
+                for (int kc = 0; kc < 16; kc++) d = Mfma<double>::run(ap[kc * 64], b[kc], d);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const double v = d[i];
+                BufIO<double>::st(rs, off, v, 128 * n * (rc + g) + 128 * ct + 32 * n * i);
+            }
+        };
+        double b0[16], b1[16];
+        Acc S[kChunk];
+        loadb(b0, 0);
+        loadb(b1, 1);
+#pragma unroll
+        for (int g = 0; g < kChunk; g++) issue(S[g], 0, g);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int jc = 0; jc < ncw; jc += 2) {
+#pragma unroll
+            for (int g = 0; g < kChunk; g++) {
+                finish(S[g], jc, g, b0);
+                __builtin_amdgcn_sched_barrier(0);
+                issue(S[g], jc + 1, g);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            loadb(b0, jc + 2);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g = 0; g < kChunk; g++) {
+                finish(S[g], jc + 1, g, b1);
+                __builtin_amdgcn_sched_barrier(0);
+                issue(S[g], jc + 2, g);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            loadb(b1, jc + 3);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+template <int RT>
+__device__ int wg_lu_rm_blocked(double *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows,
+                                int *rowcnt, int *info, Shared &sh)
+{
+    M = uniform_ptr(M);
+    n = uniform_int(n);
+    constexpr int NBC = kBatchNB;
+    const int tid = threadIdx.x;
+    typedef __attribute__((address_space(3))) double LT;
+    LT *Lp = (LT *)lp_raw;
+    if (tid == 0) st_rlx(info, 0);
+    __syncthreads();
+    for (int R0 = 0; R0 < n; R0 += kLuBlock) {
+        const int E = min(n, R0 + kLuBlock);
+        // the block's columns: its panels with row moves, U blocks and
+        // trailing updates confined to columns [R0, E)
+        for (int r0 = R0; r0 < E; r0 += NBC) {
+            const int p = r0 / NBC;
+            if (n - r0 <= NT)
+                panel_factor<1, NBC, double, true, false>(M, n, p, piv, perm_src, moved_to, rows + (size_t)p * n,
+                                                          rowcnt + p, info, sh);
+            else
+                panel_factor<kBatchMaxS / NT, NBC, double, true, false>(M, n, p, piv, perm_src, moved_to,
+                                                                        rows + (size_t)p * n, rowcnt + p, info, sh);
+            __syncthreads();
+            const int inf = ld_rlx(info);
+            if (inf) return inf;
+            for (int e = tid; e < NBC * NBC; e += NT) {
+                const int kk = e / NBC, c = e % NBC;
+                if (c < kk) sh.L11[kk][c] = M[(size_t)(r0 + kk) * n + r0 + c];
+            }
+            __syncthreads();
+            panel_rows_update<true>(M, n, r0, R0, E, perm_src, moved_to, sh);
+            __syncthreads();
+            const int c0 = r0 + NBC;
+            if (c0 < E) {
+                stage_l16<kBatchMaxS - NBC>(M, n, r0, c0, n, Lp);
+                __syncthreads();
+                tile_update16<2 * RT>(M, n, r0, c0, (n - c0) / NBC, c0, (E - c0) / NBC, Lp);
+                __syncthreads();
+            }
+        }
+        if (R0 == 0 && E == n) break;
+        // the block's row moves in every other column: its panels' moves
+        // composed (as compose_rows does) over positions [R0, n)
+        for (int i = tid; i < n; i += NT) lu_pfb[i] = i;
+        if (tid == 0) lu_nchg = 0;
+        __syncthreads();
+        for (int r0 = R0; r0 < E; r0 += NBC) {
+            int v1 = 0, v2 = 0, dst = -1;
+            if (tid < NBC) {
+                v1 = lu_pfb[perm_src[r0 + tid]];
+                v2 = lu_pfb[r0 + tid];
+                dst = moved_to[r0 + tid];
+            }
+            __syncthreads();
+            if (tid < NBC) {
+                lu_pfb[r0 + tid] = v1;
+                if (dst >= r0 + NBC) lu_pfb[dst] = v2;
+            }
+            __syncthreads();
+        }
+        for (int i = R0 + tid; i < n; i += NT) {
+            if (lu_pfb[i] != i) {
+                const int k = atomicAdd(&lu_nchg, 1);
+                if (k < 2 * kLuBlock) lu_chg[k] = i;
+            }
+        }
+        __syncthreads();
+        {
+            // 64 columns at a time: thread t takes column (t % 64) of the
+            // chunk and the changed positions t / 64 + 4 j; every source is
+            // read before any position is written
+            const int nchg = min(lu_nchg, 2 * kLuBlock);
+            const int ncol = n - (E - R0);
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+            const int cl = tid & 63, e0 = tid >> 6;
+            constexpr int kPer = 2 * kLuBlock / NWAVES;
+            int dpos[kPer], spos[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; j++) {
+                const int e = e0 + NWAVES * j;
+                dpos[j] = e < nchg ? lu_chg[e] : -1;
+                spos[j] = e < nchg ? lu_pfb[dpos[j]] : 0;
+            }
+            for (int cs = 0; cs < ncol; cs += 64) {
+                const int cc = cs + cl;
+                const int c = cc < R0 ? cc : cc + (E - R0);
+                const bool okc = cc < ncol;
+                double v[kPer];
+#pragma unroll
+                for (int j = 0; j < kPer; j++)
+                    v[j] = BufIO<double>::ld(rs, (okc && dpos[j] >= 0) ? (spos[j] * n + c) * 8 : kBufOob);
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < kPer; j++)
+                    BufIO<double>::st(rs, (okc && dpos[j] >= 0) ? (dpos[j] * n + c) * 8 : kBufOob, v[j]);
+            }
+        }
+        __syncthreads();
+        if (E == n) break;
+        // U12 of the block (rows [R0, E), columns [E, n)): per 16 rows the
+        // forward substitution, then their update of the block's later rows
+        for (int rq = R0; rq < E; rq += NBC) {
+            for (int e = tid; e < NBC * NBC; e += NT) {
+                const int kk = e / NBC, c = e % NBC;
+                if (c < kk) sh.L11[kk][c] = M[(size_t)(rq + kk) * n + rq + c];
+            }
+            __syncthreads();
+            panel_rows_update<false>(M, n, rq, E, n, perm_src, moved_to, sh);
+            __syncthreads();
+            if (rq + NBC < E) {
+                stage_l16<kLuBlock - NBC>(M, n, rq, rq + NBC, E, Lp);
+                __syncthreads();
+                tile_update16<2 * RT>(M, n, rq, rq + NBC, (E - rq - NBC) / NBC, E, (n - E) / NBC, Lp);
+                __syncthreads();
+            }
+        }
+        tile_update64(M, n, R0, E, lp_raw);
+        __syncthreads();
+    }
+    return 0;
+}
+
 template <typename T, bool BF16, int RT = 4>
 __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows, int *rowcnt,
                         int *info, Shared &sh)
 {
+    if constexpr (std::is_same<T, double>::value && !BF16) {
+        // the blocked form's extra phases and barriers cost more than the
+        // trailing-matrix traffic they save up to S = 256 (measured: 541
+        // against 506 us per LU at 256, 203 against 178 at 128; 1.76 against
+        // 2.56 ms at 512)
+        if ((n & 15) == 0 && n > kLuBlockedMin)
+            return wg_lu_rm_blocked<RT>(M, n, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+    }
     // uniform across the workgroup: scalar registers, exact buffer accesses
     M = uniform_ptr(M);
     n = uniform_int(n);

@@ -251,3 +251,70 @@ def test_batch_fp64_bit_exact_odd_sizes(gpu_lib):
         np.testing.assert_array_equal(y, o_y)
         newton += st["newton_iters"]
     assert newton > 0  # the Newton solve ran
+
+
+def _singular_at(S, col, h=0.01):
+    """S species: species `col` is autocatalytic (A -> 2A, k = 1/h, so the
+    Newton matrix entry 1 - hk is 0 at BDF step h and nothing else touches
+    column `col`); the others decay and convert into each other."""
+    from tests import cases
+    reactions, k = [], []
+    for s in range(S):
+        if s == col:
+            reactions.append(({s: 1}, {s: 2}))
+            k.append(1.0 / h)
+        else:
+            reactions.append(({s: 1}, {}))
+            k.append(0.5 + 0.01 * s)
+    rng = np.random.default_rng(3)
+    others = [s for s in range(S) if s != col]
+    for _ in range(2 * S):
+        a, b = rng.choice(others, 2, replace=False)
+        reactions.append(({int(a): 1}, {int(b): 1}))
+        k.append(float(10 ** rng.uniform(-2, 2)))
+    return cases.from_reactions(S, reactions, k, np.linspace(0.1, 1.0, S))
+
+
+def test_batch_fp64_bit_exact_blocked_lu_sizes(gpu_lib):
+    """The workgroup-level LU factors matrices with S % 16 == 0 and S > 256 in
+    64-column blocks.  Sizes that are not multiples of 64 (272 ... 496) end
+    with a 16-, 32- or 48-column block, and their trailing regions split
+    into unequal LDS chunks of row tiles; each instance of the uneven
+    generator against the oracle, bit for bit, over its first 2000 loop
+    iterations.  A 272-species model whose first BDF step meets a zero pivot
+    at column 200 (inside the fourth block; bench controller over the whole
+    horizon, faithful controller over 200 loop iterations) checks the
+    LU-failure path of the blocked factorization."""
+    sizes = (272, 304, 352, 400, 432, 496)
+    models = [synth.uneven_batch(1, 7300 + S, sizes=(S,))[0] for S in sizes]
+    t = synth.t_vector(0.0, 50.0, 100)
+    kw = dict(BENCH, max_newton=50000, max_iters=2000)
+    print(f"[batch] blocked-LU sizes {[m.S for m in models]}", flush=True)
+    dms = [gpu_lib.DeviceModel(m) for m in models]
+    outs, ys, stats, _, rc = gpu_lib.solve_batch(dms, t, raise_on_maxiter=False, **kw)
+    for dm in dms:
+        dm.close()
+    for m, out, y, st in zip(models, outs, ys, stats):
+        o_out, o_st, o_rc, _, o_y = O.solve(m, t, **kw)
+        for key in ("loop_iters", "rkf_accepts", "bdf_steps", "newton_iters", "nan_repairs",
+                    "lu_failures", "samples"):
+            assert st[key] == o_st[key], (m.S, key, st[key], o_st[key])
+        n = int(o_st["samples"])
+        np.testing.assert_array_equal(out[:n], o_out[:n])
+        np.testing.assert_array_equal(y, o_y)
+        assert st["bdf_steps"] > 0
+    m = _singular_at(272, 200)
+    t = synth.t_vector(0.0, 1.0, 11)
+    for opts, cap in ((BENCH, 20000), (dict(), 200)):
+        print(f"[batch] zero pivot at column 200 of 272, opts={opts} max_iters={cap}", flush=True)
+        dm = gpu_lib.DeviceModel(m)
+        outs, ys, stats, _, rc = gpu_lib.solve_batch([dm], t, raise_on_maxiter=False, max_iters=cap, **opts)
+        dm.close()
+        o_out, o_st, o_rc, _, o_y = O.solve(m, t, max_iters=cap, **opts)
+        assert o_st["lu_failures"] >= 1
+        for key in ("loop_iters", "rkf_accepts", "bdf_steps", "newton_iters", "nan_repairs",
+                    "lu_failures", "samples"):
+            assert stats[0][key] == o_st[key], (key, stats[0][key], o_st[key])
+        n = int(o_st["samples"])
+        np.testing.assert_array_equal(outs[0][:n], o_out[:n])
+        np.testing.assert_array_equal(ys[0], o_y)
