@@ -23,6 +23,15 @@ work (2S^3/3 per LU, 2S^2 per Newton solve) of its instances over its
 device time from HIP events on its stream, against the FP64 peak; `traffic`
 is its measured HBM bytes per launch (profiles/traffic_batch_c4.json).
 
+Differentiation leg (the discrete adjoint, one line field): 256
+parameterisations of one 512-species x 1024-reaction stiff network (k
+logU[1e-4, 1e4], one seed each), t in [0, 0.2] with 50 samples; the target
+trajectories come from the hidden rates k * U[0.5, 2] (hidden seed 7) solved
+by the batched form; one gradient evaluation = the forward pass with its step
+record + the log-loss cotangent + the reverse pass, for all 256 at once.
+Agreement of these gradients with central differences (h = 1e-6) and with
+the oracle's adjoint is checked by the GPU tests.
+
 CPU baselines (rank 0 only, before the GPU is touched, one core each):
   cpu_baseline   the oracle (the same FP64 RKF45/BDF1 algorithm on the CPU) on a
                  bounded stratified sample: full solves for S <= 128, a prefix
@@ -46,6 +55,8 @@ FP64_PEAK_TFLOPS = 78.6
 WORKLOAD = dict(n=2048, seed_base=1000, sizes=(16, 32, 64, 128, 256, 512), k_range=(1e-4, 1e4),
                 t_end=50.0, n_samples=100)
 CONTROLLER = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8, max_newton=50000)
+DIFF = dict(S=512, R=1024, B=256, seed=0, t_end=0.2, n_samples=50, eps=1e-12)
+DIFF_CTRL = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
 T_START = time.perf_counter()
 
 
@@ -61,6 +72,14 @@ def work_flops(S, st):
     """Algorithmic FP64 work of one instance: one LU per factorization
     (2S^3/3) and one pair of triangular solves per Newton iteration (2S^2)."""
     return st["lu_factorizations"] * (2.0 * S ** 3 / 3.0) + st["newton_iters"] * 2.0 * S * S
+
+
+def work_bytes(S, st):
+    """Algorithmic HBM bytes of one instance: the dense Newton matrix written
+    by the Jacobian assembly (8S^2) and written and read once by its LU
+    (16S^2) per factorization, and the factors read once per Newton solve
+    (8S^2)."""
+    return st["lu_factorizations"] * 24.0 * S * S + st["newton_iters"] * 8.0 * S * S
 
 
 def workload():
@@ -110,6 +129,49 @@ def project(models, per_size_time):
     return tot
 
 
+def differentiation_leg(B):
+    """Gradients of the log loss w.r.t. log k for B parameterisations of the
+    512 x 1024 network against targets from the hidden rates."""
+    import dataclasses
+    import numpy as np
+    from lassie_amd import _lib, synth
+    d = DIFF
+    ms, hidden = synth.parameterisations(d["S"], d["R"], B, seed=d["seed"])
+    t = synth.t_vector(0.0, d["t_end"], d["n_samples"])
+    say(f"differentiation leg: {B} parameterisations of {d['S']}x{d['R']}, targets from the hidden rates")
+    dms_h = [_lib.DeviceModel(dataclasses.replace(m, k=hidden[b])) for b, m in enumerate(ms)]
+    tgt, _, _, _, _ = _lib.solve_batch(dms_h, t, raise_on_maxiter=False, **DIFF_CTRL)
+    for dm in dms_h:
+        dm.close()
+    dms = [_lib.DeviceModel(m) for m in ms]
+    t0 = time.perf_counter()
+    outs, stats, fwd_ms, tape = _lib.forward_batch(dms, t, **DIFF_CTRL)
+    eps = d["eps"]
+    loss, W = 0.0, []
+    for y, ys in zip(outs, tgt):
+        r = np.log(y + eps) - np.log(ys + eps)
+        loss += float(np.sum(r * r))
+        W.append(2.0 * r / (y + eps))
+    gks, _, rev_ms = tape.adjoint(W, allow_nonfinite=True)
+    wall = time.perf_counter() - t0
+    finite = [tape.status(i)[0] == 1 for i in range(B)]
+    capped = sum(1 for i in range(B) if tape.status(i)[1] > 0)
+    steps = [tape.info(i)[0] for i in range(B)]
+    tape.close()
+    for dm in dms:
+        dm.close()
+    say(f"  {B} gradients in {wall:.2f} s (forward {fwd_ms / 1e3:.2f} s, reverse {rev_ms / 1e3:.2f} s), "
+        f"{sum(finite)} finite")
+    return {"gradients_per_s": B / wall, "parameterisations": B, "wall_s": wall, "forward_ms": fwd_ms,
+            "reverse_ms": rev_ms, "finite_fraction": sum(finite) / B,
+            "parameterisations_with_capped_newton_steps": capped,
+            "steps_per_parameterisation_mean": float(np.mean(steps)), "loss": loss,
+            "config": dict(d, controller=DIFF_CTRL, hidden="k * U[0.5, 2], hidden seed 7"),
+            "note": "a non-finite gradient comes from a BDF1 step whose modified Newton iteration ran all "
+                    "10000 iterations without converging; the library reports it "
+                    "(LASSIE_ERR_NONFINITE, lassie_adjoint_status); FD and oracle-adjoint agreement: GPU tests"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +185,8 @@ def main():
     ap.add_argument("--cpu-prefix", type=int, default=1500,
                     help="loop iterations of the oracle on S >= 256 sample instances")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--diff-batch", type=int, default=DIFF["B"],
+                    help="parameterisations of the differentiation leg (0 skips it)")
     ap.add_argument("--precision-batch", action="store_true",
                     help="also time the batched precision form (FP32 LU + FP64 refinement) on the "
                          "workload and report its error against the chosen form")
@@ -175,7 +239,7 @@ def main():
     # One warmup pass at most: a pass is tens of seconds of deterministic
     # work, so further warmups change nothing and the budget goes to timed
     # passes instead.
-    plain_reserve = 45.0 * args.plain_per_size + 15.0
+    plain_reserve = 45.0 * args.plain_per_size + 15.0 + (45.0 if args.diff_batch > 0 else 0.0)
     warm, first_pass = 0, None
     if args.warmup > 0:
         say("warmup pass (instances handed out largest S first)")
@@ -207,6 +271,7 @@ def main():
     wg_ms = float(np.mean([k[1] for k in kernel_ms]))
     wv_ms = float(np.mean([k[0] for k in kernel_ms]))
     achieved = wg_flops / (wg_ms / 1e3) / 1e12
+    wg_bytes = sum(work_bytes(models[i].S, stats[i]) for i in wg_idx)
 
     # plain path on a stratified sample of the same instances, bit-exact check
     plain_idx = stratified(models, args.plain_per_size)
@@ -260,7 +325,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "lassie::batch_kernel (workgroup-level, S > 64 instances)",
-                     "flops_per_launch": wg_flops, "kernel_ms": wg_ms,
+                     "flops_per_launch": wg_flops, "kernel_ms": wg_ms, "algorithmic_bytes_per_launch": wg_bytes,
                      "wave_kernel": {"instances": n_wave, "ms": wv_ms, "flops": wv_flops,
                                      "tflops": wv_flops / (wv_ms / 1e3) / 1e12 if wv_ms > 0 else None}},
         "per_size": per_size, "device_ms_per_step": dev_ms, "walls_s": walls,
@@ -275,6 +340,7 @@ def main():
             tr = json.load(f)
         line["roofline"]["traffic"] = tr.get("bytes_per_launch")
         line["roofline"]["traffic_GBps"] = tr.get("bytes_per_launch") / (wg_ms / 1e3) / 1e9
+        line["roofline"]["traffic_over_algorithmic"] = tr.get("bytes_per_launch") / wg_bytes
 
     if cpu is not None:
         # oracle: full solves below 256 species; larger ones scaled from their
@@ -327,6 +393,8 @@ def main():
     plan.close()
     for dm in dms:
         dm.close()
+    if args.diff_batch > 0:
+        line["differentiation_form"] = differentiation_leg(args.diff_batch)
     line["bench_wall_s"] = elapsed()
     print(json.dumps(line), flush=True)
 
