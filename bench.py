@@ -330,9 +330,11 @@ def main():
                                      "tflops": wv_flops / (wv_ms / 1e3) / 1e12 if wv_ms > 0 else None}},
         "per_size": per_size, "device_ms_per_step": dev_ms, "walls_s": walls,
         "schedule": {"first_pass_s": first_pass,
-                     "note": "the plan's first run hands instances out largest S first (first_pass_s, the warmup "
-                             "pass); every run records each instance's device time and the next run hands them "
-                             "out longest first, so the timed passes run the measured-cost schedule"},
+                     "wg_kernel_workgroups": int(stats[0]["groups"]) if stats else None,
+                     "note": "the plan's first run hands instances out largest S first on every CU (first_pass_s, "
+                             "the warmup pass); every run records each instance's device time, and the next run "
+                             "hands them out longest first and leaves the wave-level kernel as many CUs as balance "
+                             "the two kernels' measured work, so the timed passes run the measured-cost schedule"},
     }
     traffic = os.path.join(ROOT, "profiles", "traffic_batch_c4.json")
     if os.path.exists(traffic):

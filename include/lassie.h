@@ -179,8 +179,11 @@ int lassie_solve_batch_static(lassie_model *const *models, int32_t n_models, con
  * the next run.  The first run hands instances to the kernels' work queues
  * largest S first; each run records every instance's device time, and the
  * next run hands them out longest first (list scheduling on the measured
- * costs; LASSIE_BATCH_SCHEDULE=size keeps the by-size order).  The order
- * changes no result, only when each instance runs.  The models must outlive the plan.  lassie_batch_plan_info
+ * costs), and the workgroup-level kernel leaves the wave-level one as many
+ * CUs as balance the two kernels' measured work
+ * (LASSIE_BATCH_SCHEDULE=size keeps the by-size order and every CU for the
+ * workgroup-level kernel).  The schedule changes no result, only when and
+ * where each instance runs.  The models must outlive the plan.  lassie_batch_plan_info
  * reports the chosen variant, its row tiles and the trial times
  * (LASSIE_BATCH_VARIANTS values in ms; zeros when the choice came from the
  * cache).  Choices persist in a

@@ -10,7 +10,13 @@
 // device time (the kernels' phase timers), and the next run hands them out
 // longest first (LPT list scheduling on the measured costs), so repeated
 // runs of one plan, e.g. a parameter sweep, finish their slowest instances
-// earlier.  Rates changed
+// earlier.  The same measurements split the CUs between the two kernels: the
+// wave-level kernel's workgroups can share a CU only with each other (its
+// LDS does not fit beside a workgroup-level one), so after a run the
+// workgroup-level kernel is launched on k fewer CUs, k chosen to give both
+// kernels' measured work the same finishing time, and the wave-level kernel
+// starts on those k CUs at once instead of waiting for the other kernel to
+// drain.  Rates changed
 // with lassie_model_set_rates are used by the next run, since the graph reads
 // the models' device arrays.  Each instance follows Simulation<T>::run
 // (Simulation.cuh:632-925), as in lassie_solve_batch.
@@ -74,6 +80,9 @@ struct lassie_batch_plan {
     std::vector<int> order;
     bool measured_order = true;   // reorder by the last run's device times
     int runs = 0;
+    int wave_per_cu = 1;  // resident wave-level workgroups per CU
+    int reserve = 0;      // CUs the workgroup-level kernel leaves to the wave-level one
+    int wg_launch() const { return std::max(1, slots_for(variant) - reserve); }
     lassie::BatchArgs args{}, wargs{};
 };
 
@@ -164,6 +173,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     if (const char *cap = std::getenv("LASSIE_BATCH_SLOTS"))
         if (std::atoi(cap) > 0) P->slots = std::min(P->slots, std::atoi(cap));
     P->cus = cus;
+    P->wave_per_cu = std::max(1, wave_per_cu);
     P->wave_blocks = std::max(1, std::min(sp.n_wave, cus * wave_per_cu));
     const int smax = P->smax, slots = P->slots;
     const int n_done = slots + P->wave_blocks;  // finished-instance counters: workgroup slots, then waves
@@ -317,7 +327,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
         if (g == 0) {
             ec = lassie::launch_batch_reset(P->d_next, s, P->d_out, (size_t)P->out_tot);
         } else if (g == 1) {
-            if (sp.n_wg) ec = lassie::launch_batch(a, P->slots_for(P->variant), s, P->variant);
+            if (sp.n_wg) ec = lassie::launch_batch(a, P->wg_launch(), s, P->variant);
         } else if (g == 2) {
             if (sp.n_wave) ec = lassie::launch_wave(aw, P->wave_blocks, sp.wave_lds, s);
         } else {
@@ -338,6 +348,26 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
             return bail(LASSIE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
     *out = P;
+    return LASSIE_OK;
+}
+
+// Re-captures the workgroup-level kernel's graph with P->wg_launch() workgroups.
+static int recapture_wg(lassie_batch_plan *P)
+{
+    using lassie::api_fail;
+    hipError_t e;
+    if (P->exec[1]) (void)hipGraphExecDestroy(P->exec[1]);
+    if (P->graph[1]) (void)hipGraphDestroy(P->graph[1]);
+    P->exec[1] = nullptr;
+    P->graph[1] = nullptr;
+    if ((e = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return api_fail(LASSIE_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    const hipError_t ec = lassie::launch_batch(P->args, P->wg_launch(), P->stream, P->variant);
+    e = hipStreamEndCapture(P->stream, &P->graph[1]);
+    if (ec != hipSuccess || e != hipSuccess)
+        return api_fail(LASSIE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec != hipSuccess ? ec : e));
+    if ((e = hipGraphInstantiate(&P->exec[1], P->graph[1], nullptr, nullptr, 0)) != hipSuccess)
+        return api_fail(LASSIE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     return LASSIE_OK;
 }
 
@@ -368,7 +398,7 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
     bool maxit = false;
     for (int i = 0; i < P->n; i++) {
         const long long *c = P->h_st + (size_t)i * lassie::kBatchStats;
-        if (stats) lassie::fill_batch_stats(c, P->h_dst + 4 * (size_t)i, ms, P->slots_for(P->variant), &stats[i]);
+        if (stats) lassie::fill_batch_stats(c, P->h_dst + 4 * (size_t)i, ms, P->wg_launch(), &stats[i]);
         maxit |= c[9] == 1;
     }
     P->runs++;
@@ -384,6 +414,22 @@ extern "C" int lassie_batch_plan_run(lassie_batch_plan *P, double *out, double *
         std::stable_sort(P->order.begin() + P->n_wg, P->order.end(), longest);
         if ((e = hipMemcpy(P->d_order, P->order.data(), sizeof(int) * P->n, hipMemcpyHostToDevice)) != hipSuccess)
             return api_fail(LASSIE_ERR_HIP, std::string("plan schedule: ") + hipGetErrorString(e));
+        // CUs for the wave-level kernel: W_wg / (C - k) = W_wave / (w k)
+        // with W the measured work of each kernel's instances (CU-time for
+        // the workgroup-level kernel, wavefront-time for the wave-level one,
+        // w of whose workgroups share a CU); at most a quarter of the CUs
+        if (P->n_wg && P->n_wave) {
+            double w_wg = 0.0, w_wave = 0.0;
+            for (int j = 0; j < P->n; j++) (j < P->n_wg ? w_wg : w_wave) += (double)cost(P->order[j]);
+            int k = (int)std::lround((double)P->cus * w_wave / (P->wave_per_cu * w_wg + w_wave));
+            k = std::max(0, std::min(k, P->cus / 4));
+            k = std::min(k, P->slots_for(P->variant) - 1);
+            if (k != P->reserve) {
+                P->reserve = k;
+                int rc = recapture_wg(P);
+                if (rc != LASSIE_OK) return rc;
+            }
+        }
     }
     if (maxit)
         return api_fail(LASSIE_ERR_MAXITER,

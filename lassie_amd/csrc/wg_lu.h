@@ -399,6 +399,68 @@ __device__ void tile_update64(double *M, int n, int R0, int E, double *lds)
     }
 }
 
+// One step of block_u12_sweep: row group G (rows 16G .. 16G + 15 of the
+// block) takes the update of k = K, fma(-L[i][K], u_K, v_i) for every row
+// i > K of the group, with the L values in cur; the next step's L values go
+// into nxt first, so their LDS reads are in flight during these fmas.  The
+// steps run G ascending and, within a group, K ascending: every element
+// takes its updates in k order, and u_K is final when it is used (rows
+// below 16G are done, and row K of the group has had every k < K).
+template <int G, int K>
+struct U12Step {
+    typedef __attribute__((address_space(3))) double LT;
+    static constexpr int kRows = 16;
+    static constexpr int kLast = kRows * G + kRows - 1;  // the group's last k
+    static __device__ __forceinline__ void load(double (&l)[kRows], const LT *Lc)
+    {
+#pragma unroll
+        for (int q = 0; q < kRows; q++)
+            if (kRows * G + q > K) l[q] = Lc[K * kLuBlock + kRows * G + q];
+    }
+    static __device__ __forceinline__ void run(double (&v)[kLuBlock], const LT *Lc, double (&cur)[kRows],
+                                               double (&nxt)[kRows])
+    {
+        constexpr int Gn = K < kLast ? G : G + 1;
+        constexpr int Kn = K < kLast ? K + 1 : 0;
+        if constexpr (Gn < kLuBlock / kRows) U12Step<Gn, Kn>::load(nxt, Lc);
+        __builtin_amdgcn_sched_barrier(0);
+        const double u = v[K];
+#pragma unroll
+        for (int q = 0; q < kRows; q++)
+            if (kRows * G + q > K) v[kRows * G + q] = fma(-cur[q], u, v[kRows * G + q]);
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (Gn < kLuBlock / kRows) U12Step<Gn, Kn>::run(v, Lc, nxt, cur);
+    }
+};
+
+// U12 of a 64-row block (rows [R0, R0 + 64)) in the columns [E, n): each
+// thread takes whole columns, holds the column's 64 values in registers and
+// runs the forward substitution with the block's unit lower L11 (in LDS,
+// column-major: Lc[k * 64 + i] = L[R0 + i][R0 + k]), k ascending for every
+// element, the order the 16-row steps (substitution in the diagonal 16 x 16
+// block, MFMA update of the block's later rows) give.  No barrier: every
+// column belongs to one thread.
+__device__ __forceinline__ void block_u12_sweep(double *M, int n, int R0, int E, const double *lds)
+{
+    M = uniform_ptr(M);
+    n = uniform_int(n);
+    R0 = uniform_int(R0);
+    E = uniform_int(E);
+    typedef __attribute__((address_space(3))) double LT;
+    const LT *Lc = (const LT *)lds;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+    for (int c = E + (int)threadIdx.x; c < n; c += NT) {
+        double v[kLuBlock];
+#pragma unroll
+        for (int i = 0; i < kLuBlock; i++) v[i] = BufIO<double>::ld(rs, ((R0 + i) * n + c) * 8);
+        double la[16], lb[16];
+        U12Step<0, 0>::load(la, Lc);
+        U12Step<0, 0>::run(v, Lc, la, lb);
+#pragma unroll
+        for (int i = 0; i < kLuBlock; i++) BufIO<double>::st(rs, ((R0 + i) * n + c) * 8, v[i]);
+    }
+}
+
 template <int RT>
 __device__ int wg_lu_rm_blocked(double *__restrict__ M, int n, int *piv, int *perm_src, int *moved_to, int *rows,
                                 int *rowcnt, int *info, Shared &sh)
@@ -500,23 +562,26 @@ __device__ int wg_lu_rm_blocked(double *__restrict__ M, int n, int *piv, int *pe
         }
         __syncthreads();
         if (E == n) break;
-        // U12 of the block (rows [R0, E), columns [E, n)): per 16 rows the
-        // forward substitution, then their update of the block's later rows
-        for (int rq = R0; rq < E; rq += NBC) {
-            for (int e = tid; e < NBC * NBC; e += NT) {
-                const int kk = e / NBC, c = e % NBC;
-                if (c < kk) sh.L11[kk][c] = M[(size_t)(rq + kk) * n + rq + c];
+        // U12 of the block (rows [R0, E), columns [E, n); E = R0 + 64 here):
+        // the block's L11 to LDS, then one register sweep per column
+        {
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
+            constexpr int kPer = kLuBlock * kLuBlock / NT;
+            double v[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; j++) {
+                const int e = tid + j * NT, i = e / kLuBlock, k = e % kLuBlock;
+                v[j] = BufIO<double>::ld(rs, ((R0 + i) * n + R0 + k) * 8);
             }
-            __syncthreads();
-            panel_rows_update<false>(M, n, rq, E, n, perm_src, moved_to, sh);
-            __syncthreads();
-            if (rq + NBC < E) {
-                stage_l16<kLuBlock - NBC>(M, n, rq, rq + NBC, E, Lp);
-                __syncthreads();
-                tile_update16<2 * RT>(M, n, rq, rq + NBC, (E - rq - NBC) / NBC, E, (n - E) / NBC, Lp);
-                __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kPer; j++) {
+                const int e = tid + j * NT, i = e / kLuBlock, k = e % kLuBlock;
+                Lp[k * kLuBlock + i] = v[j];
             }
         }
+        __syncthreads();
+        block_u12_sweep(M, n, R0, E, lp_raw);
+        __syncthreads();
         tile_update64(M, n, R0, E, lp_raw);
         __syncthreads();
     }
