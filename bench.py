@@ -172,6 +172,44 @@ def differentiation_leg(B):
                     "(LASSIE_ERR_NONFINITE, lassie_adjoint_status); FD and oracle-adjoint agreement: GPU tests"}
 
 
+def precision_form_4096(cap):
+    """The survey's precision configuration: the 4096 x 4096 stiff model
+    (k logU[1e-6, 1e6], x0 logU[1e-6, 1], seed 0, t in [0, 100], 200 samples)
+    in the plain FP64 form and in the single-model precision form (FP32 grid
+    LU on the matrix cores, FP64 refinement of every Newton correction to a
+    componentwise backward error with the species' atol as floor), over the
+    first `cap` loop iterations of the same controller; phase times per
+    factorization and per Newton iteration, and the error of the final state
+    against the FP64 form."""
+    import numpy as np
+    from lassie_amd import _lib, synth
+    m = synth.stiff(4096, 4096, 0, (1e-6, 1e6))
+    t = synth.t_vector(0.0, 100.0, 200)
+    ctrl = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
+    dm = _lib.DeviceModel(m)
+    res, finals = {}, {}
+    for name, kw in (("fp64", {}), ("fp32_refine", dict(precision=_lib.PREC_FP32_REFINE))):
+        say(f"  4096x4096 {name}, first {cap} loop iterations")
+        t0 = time.perf_counter()
+        _, yf, st, rc, _ = dm.solve(t, raise_on_maxiter=False, max_iters=cap, **ctrl, **kw)
+        wall = time.perf_counter() - t0
+        finals[name] = yf
+        lu = max(st["lu_factorizations"], 1)
+        res[name] = {"wall_s": wall, "kernel_ms": st["kernel_ms"], "bdf_steps": st["bdf_steps"],
+                     "newton_iters": st["newton_iters"], "lu_factorizations": st["lu_factorizations"],
+                     "lu_ms_per_factorization": st["phase_ms"][2] / lu,
+                     "newton_ms_per_iteration": st["phase_ms"][3] / max(st["newton_iters"], 1),
+                     "refine_sweeps": st["refine_iters"], "refine_fallbacks": st["refine_fallbacks"],
+                     "groups": st["groups"]}
+        say(f"  {name}: {wall:.1f} s, LU {res[name]['lu_ms_per_factorization']:.2f} ms, Newton iteration "
+            f"{res[name]['newton_ms_per_iteration']:.3f} ms")
+    dm.close()
+    a, b = finals["fp64"], finals["fp32_refine"]
+    res["max_rel_err_final_state_vs_fp64"] = float(np.max(np.abs(b - a) / (np.abs(a) + m.atol)))
+    res["loop_iterations"] = cap
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +225,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--diff-batch", type=int, default=DIFF["B"],
                     help="parameterisations of the differentiation leg (0 skips it)")
+    ap.add_argument("--precision-4096", type=int, default=0, metavar="N",
+                    help="also compare the plain and precision forms on the 4096x4096 model over its first N "
+                         "loop iterations (not part of the default run)")
     ap.add_argument("--precision-batch", action="store_true",
                     help="also time the batched precision form (FP32 LU + FP64 refinement) on the "
                          "workload and report its error against the chosen form")
@@ -397,6 +438,9 @@ def main():
         dm.close()
     if args.diff_batch > 0:
         line["differentiation_form"] = differentiation_leg(args.diff_batch)
+    if args.precision_4096 > 0:
+        say(f"precision form on 4096x4096, {args.precision_4096} loop iterations")
+        line["precision_form_4096"] = precision_form_4096(args.precision_4096)
     line["bench_wall_s"] = elapsed()
     print(json.dumps(line), flush=True)
 

@@ -54,14 +54,17 @@ __device__ void strip_moves(T *__restrict__ M, int n, int p, int q, const int *p
 // update) to strip q (columns [q*NBC, q*NBC + ncols)), q > p.  One group.
 // M must be followed by kLuSink writable elements.
 //
-// With 16-column FP64 panels the trailing update A22 -= L21 U12 runs on the matrix
+// With 16-column panels the trailing update A22 -= L21 U12 runs on the matrix
 // cores over every row below the panel, as the oracle does: 16-row tiles,
 // computed transposed (A22^T -= U12^T L21^T) so that the lanes of a 16-lane
-// group hold consecutive rows of the column-major matrix, four 16x16x4 FP64
-// MFMAs per tile with k ascending (the instruction accumulates as a
-// sequential fma chain, the oracle's order).  A wave walks the row tiles
-// wave, wave+4, ... RT at a time with the next RT tiles' operands in flight.
-// Otherwise (8-column panels, n > 2048, or float) the update uses vector fmas over the rows
+// group hold consecutive rows of the column-major matrix, four 16x16x4
+// MFMAs per tile (FP64, or FP32 for the single-precision loop and the FP32
+// factors of the precision form) with k ascending: both instructions
+// accumulate as a sequential fma chain, the oracle's order (FP32: equal to
+// an fmaf chain on 1048576 of 1048576 random elements over 20 binades).  A
+// wave walks the row tiles wave, wave+4, ... RT at a time with the next RT
+// tiles' operands in flight.
+// Otherwise (8-column panels) the update uses vector fmas over the rows
 // listed by the panel (all others have zero multipliers, so
 // a_ij = fma(-0, u, a_ij) = a_ij), or every row when the U block holds a
 // non-finite value.
@@ -118,7 +121,7 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
         }
     }
     // the 8-column update visits every row when U12 holds a non-finite value
-    constexpr bool kMfma = NBC == 16 && sizeof(T) == 8;
+    constexpr bool kMfma = NBC == 16;
     bool all_rows = false;
     if constexpr (kMfma)
         __syncthreads();
@@ -130,10 +133,10 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
         (void)rows;
         (void)rowcnt;
         constexpr int RT = 4;
-        typedef Mfma<double>::acc Acc;
+        typedef typename Mfma<T>::acc Acc;
         // global-memory view: precise memory counters keep the next tiles'
         // loads in flight across the current MFMAs
-        typedef __attribute__((address_space(1))) double GD;
+        typedef __attribute__((address_space(1))) T GD;
         GD *Mg = (GD *)M;
         const size_t last = (size_t)n * n - 1;
         const size_t sink = (size_t)n * n;
@@ -141,38 +144,38 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
         const int rb = r0 + nbp;             // first row below the panel
         const int nt = (n - rb + 15) >> 4;   // row tiles
         // A operand: U12^T (A[c][k] = U12[k][c]); padding columns are 0
-        double ua[4];
+        T ua[4];
 #pragma unroll
         for (int kc = 0; kc < 4; kc++) {
             const int k = 4 * kc + lq;
-            ua[kc] = (k < nbp && l16 < ncols) ? sh.U12[k][l16] : 0.0;
+            ua[kc] = (k < nbp && l16 < ncols) ? (T)sh.U12[k][l16] : T(0);
         }
         // tile t of this wave: rows rb + 16 (wave + NWAVES t) + l16
-        auto load_group = [&](int g, Acc *dd, double (*bb)[4]) {
+        auto load_group = [&](int g, Acc *dd, T (*bb)[4]) {
 #pragma unroll
             for (int u = 0; u < RT; u++) {
                 const int r = rb + 16 * (wave + NWAVES * (g + u)) + l16;
                 const bool okr = r < n;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const int c = Mfma<double>::row(lq, i);
+                    const int c = Mfma<T>::row(lq, i);
                     const bool ok = okr && c < ncols;
-                    const double v = Mg[ok ? (size_t)(c0 + c) * n + r : last];
-                    dd[u][i] = ok ? v : 0.0;
+                    const T v = Mg[ok ? (size_t)(c0 + c) * n + r : last];
+                    dd[u][i] = ok ? v : T(0);
                 }
 #pragma unroll
                 for (int kc = 0; kc < 4; kc++) {
                     const int k = 4 * kc + lq;
                     const bool ok = okr && k < nbp;
-                    const double v = Mg[ok ? (size_t)(r0 + k) * n + r : last];
+                    const T v = Mg[ok ? (size_t)(r0 + k) * n + r : last];
                     // padding products are 0 * (-0) = -0, which leaves every value unchanged
-                    bb[u][kc] = ok ? -v : -0.0;
+                    bb[u][kc] = ok ? -v : T(-0.0);
                 }
             }
         };
         const int ntw = (nt - wave + NWAVES - 1) / NWAVES;  // tiles of this wave
         Acc d[RT], dn[RT];
-        double b[RT][4], bn[RT][4];
+        T b[RT][4], bn[RT][4];
         load_group(0, d, b);
         for (int g = 0; g < ntw; g += RT) {
             const bool more = g + RT < ntw;
@@ -180,14 +183,14 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
 #pragma unroll
             for (int u = 0; u < RT; u++) {
 #pragma unroll
-                for (int kc = 0; kc < 4; kc++) d[u] = Mfma<double>::run(ua[kc], b[u][kc], d[u]);
+                for (int kc = 0; kc < 4; kc++) d[u] = Mfma<T>::run(ua[kc], b[u][kc], d[u]);
             }
 #pragma unroll
             for (int u = 0; u < RT; u++) {
                 const int r = rb + 16 * (wave + NWAVES * (g + u)) + l16;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const int c = Mfma<double>::row(lq, i);
+                    const int c = Mfma<T>::row(lq, i);
                     const bool ok = g + u < ntw && r < n && c < ncols;
                     Mg[ok ? (size_t)(c0 + c) * n + r : sink] = d[u][i];
                 }
@@ -268,7 +271,7 @@ __device__ void panel_fit(const Store st, int n, int p, int *piv, int *perm_src,
         }
     }
     // row lists feed the vector strip update only
-    constexpr bool kList = !(NBC == 16 && sizeof(T) == 8);
+    constexpr bool kList = NBC != 16;  // row lists feed the vector (8-column) update only
     panel_factor_s<RPT, NBC, T, kList>(st, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
 }
 
@@ -469,7 +472,7 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
         return true;
     }
     for (int p = 0; p < P; p++) {
-        const int cnt = (NBC == 16 && sizeof(T) == 8) ? 0 : A.rowcnt[p];  // row lists feed the vector update only
+        const int cnt = NBC == 16 ? 0 : A.rowcnt[p];  // row lists feed the vector update only
         const int *rows = A.rows + (size_t)p * n;
         int q = p + 1 + ((wg - (p + 1)) % G + G) % G;
         for (; q < P; q += G) {
@@ -773,6 +776,11 @@ template <int RPT, int NBC, typename T, bool REF = false>
 __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
 {
     static_assert(!REF || sizeof(T) == 8, "the precision form runs the FP64 loop");
+    // panel width of the precision form's FP32 factors: NBC, as for the FP64
+    // LU.  (16-column FP32 panels above n = 2048 hold 16 rows x 16 columns
+    // per thread and spill: 52 ms per 4096 x 4096 factorization, against
+    // 28 ms with 8-column panels and the vector update.)
+    constexpr int kNbc32 = NBC;
     extern __shared__ __align__(16) unsigned char dyn[];
     __shared__ Shared sh;
     const int tid = threadIdx.x;
@@ -973,7 +981,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 SolveArgs A32 = A;
                 A32.M = reinterpret_cast<double *>(A.M32);
                 A32.lds_strip = 0;
-                if (!lu_factor_grid<RPT, NBC, float>(A32, wg, sh, gen_local, singular)) return;
+                if (!lu_factor_grid<RPT, kNbc32, float>(A32, wg, sh, gen_local, singular)) return;
                 fp64 = singular;
                 if (singular) {
                     // zero FP32 pivot: the FP64 LU of M for this step.  Every
@@ -1009,7 +1017,14 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 A32.M = reinterpret_cast<double *>(A.M32);
                 if (wg == 0) {
                     st_lu++;
-                    if (!singular) compose_rows<NBC>(A.perm_src, A.moved_to, n, A.perm_full);
+                    // the row order of the current factors: FP32 (kNbc32-column
+                    // panels), or FP64 after a zero FP32 pivot
+                    if (!singular) {
+                        if (fp64)
+                            compose_rows<NBC>(A.perm_src, A.moved_to, n, A.perm_full);
+                        else
+                            compose_rows<kNbc32>(A.perm_src, A.moved_to, n, A.perm_full);
+                    }
                 }
                 int itN = 0;
                 T errN = 1000;
