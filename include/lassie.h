@@ -46,11 +46,12 @@ extern "C" {
  * target of refine_tol, but trajectories are not within 1e-6 of the FP64
  * method's (the loop's step decisions are discontinuous; per-instance median
  * relative errors of about 1e-3 to 5e-2 on the stiff batched workloads).
- * BF16 is emulated: the trailing-update operands are rounded to BF16 and
- * multiplied on the FP32 matrix cores, with FP32 accumulation. */
+ * BF16: the panel factorizations run in FP32 and the trailing updates on
+ * the BF16 matrix cores (operands rounded to BF16, one 16x16x16 BF16 MFMA
+ * per 16 x 16 tile of a 16-column panel, FP32 accumulation). */
 #define LASSIE_PREC_FP64 0          /* plain method: FP64 LU (cuSOLVER getrf/getrs seat) */
 #define LASSIE_PREC_FP32_REFINE 1   /* FP32 LU + FP64 iterative refinement (experimental) */
-#define LASSIE_PREC_BF16_REFINE 2   /* BF16-rounded LU + FP64 iterative refinement (experimental, batched only) */
+#define LASSIE_PREC_BF16_REFINE 2   /* BF16 LU updates + FP64 iterative refinement (experimental, batched only) */
 /* the whole loop in single precision: `-p float`, Simulation<float>
  * (simulator.cu:259-304); lassie_solve only */
 #define LASSIE_PREC_FP32_LOOP 3

@@ -7,8 +7,8 @@
 // Newton linear solve per instance, in the workgroup, row-major dense M:
 //   precision FP64   partial-pivot LU in FP64, bit-identical to the oracle
 //   FP32 / BF16      M is rounded to FP32 and factored with the same pivoting
-//                    in FP32 (BF16: trailing-update operands rounded to BF16,
-//                    FP32 accumulation); each Newton correction is refined in
+//                    in FP32 (BF16: the trailing updates on the BF16 matrix
+//                    cores, FP32 accumulation); each Newton correction is refined in
 //                    FP64 against the FP64 M to a componentwise backward error
 //                    of refine_tol, else the step falls back to the FP64 LU.
 #include "batch.h"

@@ -21,7 +21,8 @@ __device__ __forceinline__ float to_bf16_rne(float x)
 // partial pivoting (panel_factor semantics).  M must be followed by kLuSink
 // writable elements.  RT: row tiles of the trailing update in flight per wave.  Row moves go to every column,
 // so on return M holds L\U in final row order.  Returns 0 or k+1 of the
-// first zero pivot.  BF16 = round the trailing-update operands to BF16.
+// first zero pivot.  BF16: the trailing update on the BF16 matrix cores
+// (operands rounded to BF16, FP32 accumulation).
 constexpr int kLpStride = 17;  // LDS row stride of the staged L21 panel (elements)
 
 // L21 of the current panel (at most kBatchMaxS - kBatchNB rows), shared by
@@ -736,6 +737,59 @@ __device__ int wg_lu_rm(T *__restrict__ M, int n, int *piv, int *perm_src, int *
             for (int ct = wave; ct < nt; ct += NWAVES) {
                 const int col = c0 + 16 * ct + l16;
                 const bool okc = col < n;
+                if constexpr (BF16) {
+                    // one v_mfma_f32_16x16x16_bf16 per tile (k = 0..15, the
+                    // whole panel): lane l holds A[l%16][4(l/16)+j] and
+                    // B[4(l/16)+j][l%16], j < 4, as BF16; FP32 accumulation
+                    // with the FP32 instruction's output layout (measured)
+                    typedef short S4 __attribute__((ext_vector_type(4)));
+                    auto bits16 = [](float x) { return (short)(__float_as_uint(x) >> 16); };
+                    S4 b4;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int k = 4 * lq + j;
+                        const bool ok = okc && k < nbp;
+                        const T bv = Mg[ok ? (size_t)(r0 + k) * n + col : 0];
+                        b4[j] = bits16(ok ? to_bf16_rne((float)bv) : 0.f);
+                    }
+                    for (int g = 0; g < nt; g += RT) {
+                        Acc d[RT];
+                        S4 a4[RT];
+#pragma unroll
+                        for (int u = 0; u < RT; u++) {
+                            const int R = c0 + 16 * (g + u);
+                            const bool okt = g + u < nt;
+#pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                const int r = R + Mfma<T>::row(lq, i);
+                                const bool ok = okt && r < n && okc;
+                                const T v = Mg[ok ? (size_t)r * n + col : last];
+                                d[u][i] = ok ? v : T(0);
+                            }
+                            const int ra = R + l16;
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                const int k = 4 * lq + j;
+                                const bool ok = okt && ra < n && k < nbp;
+                                // staged negated and rounded to BF16; padding -0 * 0 = -0
+                                a4[u][j] = bits16(ok ? (float)Lp[(ra - c0) * kLpStride + k] : -0.f);
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < RT; u++) d[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a4[u], b4, d[u], 0, 0, 0);
+#pragma unroll
+                        for (int u = 0; u < RT; u++) {
+                            const int R = c0 + 16 * (g + u);
+#pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                const int r = R + Mfma<T>::row(lq, i);
+                                const bool ok = g + u < nt && r < n && okc;
+                                Mg[ok ? (size_t)r * n + col : sink] = d[u][i];
+                            }
+                        }
+                    }
+                    continue;
+                }
                 T b[4];
 #pragma unroll
                 for (int kc = 0; kc < 4; kc++) {
