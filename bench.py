@@ -173,10 +173,10 @@ def differentiation_leg(B):
 
 
 def precision_form_4096(cap):
-    """The survey's precision configuration: the 4096 x 4096 stiff model
+    """The FP32-refinement configuration: the 4096 x 4096 stiff model
     (k logU[1e-6, 1e6], x0 logU[1e-6, 1], seed 0, t in [0, 100], 200 samples)
     in the plain FP64 form and in the single-model precision form (FP32 grid
-    LU on the matrix cores, FP64 refinement of every Newton correction to a
+    LU with 8-column panels, FP64 refinement of every Newton correction to a
     componentwise backward error with the species' atol as floor), over the
     first `cap` loop iterations of the same controller; phase times per
     factorization and per Newton iteration, and the error of the final state
