@@ -2,6 +2,7 @@
 // solvers: LDS scratch, fixed-order reductions, the RKF45 attempt and the
 // Jacobian row assembly.  Same operation order as the reference kernels.
 #pragma once
+#include <type_traits>
 #include "device_common.h"
 
 namespace lassie {
@@ -285,7 +286,7 @@ __device__ int rkf_attempt_wg(const DevModel &m, double rtol, const CtlViewT<T> 
 // T is the state and matrix type; each term is accumulated in double and
 // added to the T entry, and m.hk holds sign * k formed in T.
 template <bool ROW_MAJOR, typename T>
-__device__ void jac_row(const DevModel &m, const T *__restrict__ x, T *__restrict__ M, int n, int i, T h)
+__device__ void jac_row_rmw(const DevModel &m, const T *__restrict__ x, T *__restrict__ M, int n, int i, T h)
 {
     auto at = [&](int row, int col) -> T * {
         return ROW_MAJOR ? M + (size_t)row * n + col : M + (size_t)col * n + row;
@@ -336,6 +337,81 @@ __device__ void jac_row(const DevModel &m, const T *__restrict__ x, T *__restric
             *p = (T)0.0 - h * *p;
         }
     }
+}
+
+// Row i of M = I - h J (createJac_ and aggiorna_Jacobiano1, kernels.cuh:205-271,
+// 393-402) with the row's entries accumulated in registers: each column
+// takes the same additions in the same order as jac_row_rmw's global
+// read-modify-writes (starting from the zeroed matrix's +0), then one store
+// per entry: 1 - h v on the diagonal (always stored), 0 - h v elsewhere.  M
+// must be zero in row i.  A row with more than kJacCols distinct columns
+// takes jac_row_rmw.
+constexpr int kJacCols = 16;
+
+template <bool ROW_MAJOR, typename T>
+__device__ void jac_row(const DevModel &m, const T *__restrict__ x, T *__restrict__ M, int n, int i, T h)
+{
+    auto at = [&](int row, int col) -> T * {
+        return ROW_MAJOR ? M + (size_t)row * n + col : M + (size_t)col * n + row;
+    };
+    int col[kJacCols];
+    T val[kJacCols];
+#pragma unroll
+    for (int k = 0; k < kJacCols; k++) {
+        col[k] = -1;
+        val[k] = T(0);
+    }
+    int cnt = 0;
+    bool over = false;
+    const int e0 = m.hptr[i], e1 = m.hptr[i + 1];
+    for (int e = e0; e < e1 && !over; e++) {
+        const int r = m.hrx[e];
+        const T sk = (T)m.hk[e];
+        const int a0 = m.lptr[r], a1 = m.lptr[r + 1];
+        for (int j = a0; j < a1; j++) {
+            const int g = m.lsp[j];
+            const int ex = m.lexp[j];
+            double temp = 0.0;
+            if (ex == 1)
+                temp = temp + sk;
+            else if (ex > 1)
+                temp = temp + (sk * (T)ex) * mypow_t(x[g], ex - 1);
+            else
+                continue;
+            for (int jj = a0; jj < a1; jj++)
+                if (m.lsp[jj] != g) temp = temp * x[m.lsp[jj]];
+            int hit = -1;
+#pragma unroll
+            for (int k = 0; k < kJacCols; k++) hit = (col[k] == g) ? k : hit;
+            if (hit < 0) {
+                if (cnt == kJacCols) {
+                    over = true;
+                    break;
+                }
+                hit = cnt++;
+            }
+            T cur = T(0);
+#pragma unroll
+            for (int k = 0; k < kJacCols; k++) cur = (k == hit) ? val[k] : cur;
+            const T nv = (T)(cur + temp);
+#pragma unroll
+            for (int k = 0; k < kJacCols; k++) {
+                val[k] = (k == hit) ? nv : val[k];
+                col[k] = (k == hit) ? g : col[k];
+            }
+        }
+    }
+    if (over) {
+        jac_row_rmw<ROW_MAJOR>(m, x, M, n, i, h);
+        return;
+    }
+    T dv = T(0);
+#pragma unroll
+    for (int k = 0; k < kJacCols; k++) {
+        if (k < cnt && col[k] != i) *at(i, col[k]) = (T)0.0 - h * val[k];
+        dv = (k < cnt && col[k] == i) ? val[k] : dv;
+    }
+    *at(i, i) = (T)1.0 - h * dv;
 }
 
 // ---------------------------------------------------------------------------
@@ -941,14 +1017,16 @@ __device__ void lu_solve_rows_impl(const double *__restrict__ M, int n, double *
                 ld(&X.v[q][j], r, k0 + j, row_ok && k0 + j + VEC - 1 >= r && k0 + j < n);
         }
     };
-    auto stepF = [&](Set &X, int c) {
-        const int k0 = c * TB, kb = min(TB, n - k0);
+    auto k0_full = [&](int c) { return c * TB + TB <= n; };
+    auto stepF_ = [&](Set &X, int c, auto full) {
+        constexpr bool FULL = decltype(full)::value;
+        const int k0 = c * TB, kb = FULL ? TB : min(TB, n - k0);
         const int q = k0 / NT, dw = (k0 % NT) / 64, l0 = k0 % 64;
         if (wave == dw) {
             auto chain = [&](double &a, const double(&L)[TB]) {
 #pragma unroll
                 for (int kk = 0; kk < TB; kk++) {
-                    if (kb == TB || kk < kb) {
+                    if (FULL || kk < kb) {
                         const double xk = readlane_d(a, l0 + kk);
                         if (lane > l0 + kk && lane < l0 + kb) a = fma(-L[kk], xk, a);
                     }
@@ -970,18 +1048,27 @@ __device__ void lu_solve_rows_impl(const double *__restrict__ M, int n, double *
             if (r < n && r >= k0 + kb) {
 #pragma unroll
                 for (int kk = 0; kk < TB; kk++)
-                    if (kb == TB || kk < kb) acc[q2] = fma(-X.v[q2][kk], xs[kk], acc[q2]);
+                    if (FULL || kk < kb) acc[q2] = fma(-X.v[q2][kk], xs[kk], acc[q2]);
             }
         }
     };
-    auto stepB = [&](Set &X, int c) {
-        const int k0 = c * TB, kb = min(TB, n - k0);
+    // a full block (every block when TB divides n) without per-column checks,
+    // which otherwise become selects inside the fma chains
+    auto stepF = [&](Set &X, int c) {
+        if (k0_full(c))
+            stepF_(X, c, std::true_type{});
+        else
+            stepF_(X, c, std::false_type{});
+    };
+    auto stepB_ = [&](Set &X, int c, auto full) {
+        constexpr bool FULL = decltype(full)::value;
+        const int k0 = c * TB, kb = FULL ? TB : min(TB, n - k0);
         const int q = k0 / NT, dw = (k0 % NT) / 64, l0 = k0 % 64;
         if (wave == dw) {
             auto chain = [&](double &a, const double(&U)[TB]) {
 #pragma unroll
                 for (int kk = TB - 1; kk >= 0; kk--) {
-                    if (kb == TB || kk < kb) {
+                    if (FULL || kk < kb) {
                         if (lane == l0 + kk) a = a / U[kk];
                         const double xk = readlane_d(a, l0 + kk);
                         if (lane >= l0 && lane < l0 + kk) a = fma(-U[kk], xk, a);
@@ -1004,9 +1091,15 @@ __device__ void lu_solve_rows_impl(const double *__restrict__ M, int n, double *
             if (r < k0) {
 #pragma unroll
                 for (int kk = TB - 1; kk >= 0; kk--)
-                    if (kb == TB || kk < kb) acc[q2] = fma(-X.v[q2][kk], xs[kk], acc[q2]);
+                    if (FULL || kk < kb) acc[q2] = fma(-X.v[q2][kk], xs[kk], acc[q2]);
             }
         }
+    };
+    auto stepB = [&](Set &X, int c) {
+        if (k0_full(c))
+            stepB_(X, c, std::true_type{});
+        else
+            stepB_(X, c, std::false_type{});
     };
     // the scheduling barriers keep each set's loads together and in program
     // order, so the wait for set c leaves set c + 1 in flight

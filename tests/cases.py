@@ -62,3 +62,19 @@ def with_feed():
     """A + B -> C with B held at a feed value."""
     return from_reactions(3, [({0: 1, 1: 1}, {2: 1}), ({2: 1}, {0: 1})], [1.2, 0.4],
                           [1.0, 0.5, 0.0], feed=[0.0, 0.5, 0.0])
+
+
+def hub(partners=20):
+    """A + B_k <-> C_k for k = 1..partners: the Jacobian row of A has
+    2 partners + 1 distinct columns (more than the register-accumulated
+    row form holds), the other rows three."""
+    S = 1 + 2 * partners
+    reactions, k = [], []
+    for i in range(partners):
+        b, c = 1 + i, 1 + partners + i
+        reactions.append(({0: 1, b: 1}, {c: 1}))
+        k.append(200.0 * (1 + i))
+        reactions.append(({c: 1}, {0: 1, b: 1}))
+        k.append(5.0 + i)
+    x0 = [1.0] + [0.5] * partners + [0.0] * partners
+    return from_reactions(S, reactions, k, x0)

@@ -318,3 +318,28 @@ def test_batch_fp64_bit_exact_blocked_lu_sizes(gpu_lib):
         n = int(o_st["samples"])
         np.testing.assert_array_equal(outs[0][:n], o_out[:n])
         np.testing.assert_array_equal(ys[0], o_y)
+
+
+def test_batch_hub_rows_bit_exact(gpu_lib):
+    """Jacobian rows with more distinct columns than the register-accumulated
+    form holds (the hub species of cases.hub) take the read-modify-write
+    form: both kernels against the oracle, bit for bit."""
+    from tests import cases
+    models = [cases.hub(20), cases.hub(40)]  # S = 41 (wave-level), 81 (workgroup-level)
+    t = synth.t_vector(0.0, 5.0, 21)
+    kw = dict(BENCH, max_iters=200_000)
+    print(f"[batch] hub models {[m.S for m in models]}", flush=True)
+    dms = [gpu_lib.DeviceModel(m) for m in models]
+    outs, ys, stats, _, rc = gpu_lib.solve_batch(dms, t, raise_on_maxiter=False, **kw)
+    for dm in dms:
+        dm.close()
+    for m, out, y, st in zip(models, outs, ys, stats):
+        o_out, o_st, o_rc, _, o_y = O.solve(m, t, **kw)
+        assert o_st["bdf_steps"] > 0
+        for key in ("loop_iters", "rkf_accepts", "bdf_steps", "newton_iters", "nan_repairs", "lu_failures",
+                    "samples"):
+            assert st[key] == o_st[key], (m.S, key, st[key], o_st[key])
+        n = int(o_st["samples"])
+        np.testing.assert_array_equal(out[:n], o_out[:n])
+        np.testing.assert_array_equal(y, o_y)
+

@@ -54,7 +54,7 @@ def test_small_nonstiff_32x48(gpu_lib, opts):
 
 
 @pytest.mark.parametrize("name", ["decay", "reversible", "chain", "dimer", "mixed", "with_feed",
-                                  "autocatalytic_singular"])
+                                  "autocatalytic_singular", "hub"])
 @pytest.mark.parametrize("opts", [FAITHFUL, BENCH], ids=["faithful", "bench"])
 def test_hand_models(gpu_lib, name, opts):
     m = getattr(cases, name)()

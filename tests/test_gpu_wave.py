@@ -55,7 +55,7 @@ def test_wave_ragged_sizes_bit_exact(gpu_lib, ctrl):
 
 def test_wave_hand_models_bit_exact(gpu_lib):
     models = [cases.decay(), cases.reversible(), cases.chain(), cases.dimer(), cases.mixed(),
-              cases.autocatalytic_singular(), cases.with_feed()]
+              cases.autocatalytic_singular(), cases.with_feed(), cases.hub(20)]
     t = synth.t_vector(0, 5, 11)
     for kw in (dict(BENCH, max_iters=200_000), dict(max_iters=50_000)):
         outs, ys, stats, rc = _run(models, t, **kw)
