@@ -28,6 +28,7 @@
 //                           (Simulation.cuh:58-64, 883-888)
 //   step-factor minimum     thrust::min_element semantics (Simulation.cuh:753-754)
 #include "device_common.h"
+#include <utility>
 #include "wave.h"
 #include "wg_common.h"
 
@@ -159,65 +160,6 @@ __device__ int rkf_attempt_wave(const DevModel &m, const HRec *recs, const int *
     return nok;
 }
 
-// LU with partial pivoting of the S x S row-major matrix M (leading dimension
-// ld) in LDS, rows physically swapped as in getrf.  myorig receives, in lane
-// i, the original row now at position i.  Returns 0, or k + 1 at the first
-// zero pivot (the matrix is then singular for the loop, Simulation.cuh:1309).
-// The trailing update covers 64 / NPC rows per pass (lane -> row lane / NPC,
-// column lane % NPC), NPC >= S, with four passes' loads issued together.
-template <int NPC>
-__device__ int lu_wave(double *M, int ld, int S, int &myorig)
-{
-    constexpr int RPP = WL / NPC;
-    constexpr int UB = 4;
-    const int lane = threadIdx.x;
-    const int prow = lane / NPC, pcol = lane % NPC;
-    myorig = lane;
-    for (int k = 0; k < S; k++) {
-        // pivot: the largest |a| of rows k..S-1 in column k, first on ties; a
-        // NaN on the diagonal keeps the diagonal, NaNs below never win
-        const bool cand = (lane >= k) & (lane < S);
-        const double a = cand ? M[lane * ld + k] : 0.0;
-        const double v = cand ? (isnan(a) ? ((lane == k) ? INFINITY : -1.0) : fabs(a)) : -2.0;
-        double wv;
-        int p;
-        wave_argmax(v, cand ? lane : 0x7fffffff, wv, p);
-        const double ukk = readlane_d(a, p);
-        if (ukk == 0.0) return k + 1;
-        if (p != k) {
-            if (lane < S) {
-                const double rk = M[k * ld + lane], rp = M[p * ld + lane];
-                M[k * ld + lane] = rp;
-                M[p * ld + lane] = rk;
-            }
-            const int ok = __shfl(myorig, k), op = __shfl(myorig, p);
-            myorig = (lane == k) ? op : ((lane == p) ? ok : myorig);
-        }
-        __syncthreads();
-        if ((lane > k) & (lane < S)) M[lane * ld + k] = M[lane * ld + k] / ukk;
-        __syncthreads();
-        const bool colok = (pcol > k) & (pcol < S);
-        const double u = colok ? M[k * ld + pcol] : 0.0;
-        for (int i0 = k + 1; i0 < S; i0 += UB * RPP) {
-            double l[UB], x[UB];
-#pragma unroll
-            for (int q = 0; q < UB; q++) {
-                const int i = i0 + q * RPP + prow;
-                const bool ok = colok & (i < S);
-                l[q] = ok ? M[i * ld + k] : 0.0;
-                x[q] = ok ? M[i * ld + pcol] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < UB; q++) {
-                const int i = i0 + q * RPP + prow;
-                if (colok & (i < S)) M[i * ld + pcol] = fma(-l[q], u, x[q]);
-            }
-        }
-        __syncthreads();
-    }
-    return 0;
-}
-
 // getrs with one right-hand side: b = rhs permuted by the factors' row order
 // (lane i takes rhs[myorig]), forward substitution with the unit lower factor
 // (k ascending), backward with the upper factor (k descending), lane i
@@ -265,11 +207,87 @@ __device__ double lu_solve_wave(const double *M, int ld, int S, int myorig, cons
     return b;
 }
 
+// LU with partial pivoting of the S x S row-major matrix M (leading dimension
+// ld, LDS), getrf's row swaps, with the matrix in registers: lane i holds row
+// i (NPC doubles) for the whole factorization and rows never move; each
+// row's position (pos) follows the swaps, so the pivot search (the largest
+// |a| among the rows at positions k.., the lowest position on ties; a NaN at
+// position k keeps the diagonal, NaNs below never win) and every row's
+// operations are those of the physical swaps.  Per column: the pivot's lane
+// by ballot, the multipliers by division, the pivot row's later entries by
+// readlane, one fma per entry; the factors go to M at their final positions.
+// myorig receives, in lane i, the original row now at position i.  Returns
+// 0, or k + 1 at the first zero pivot (the matrix is then singular for the
+// loop, Simulation.cuh:1309).  FULL: S == NPC (no column checks).
+template <typename F, int... K>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, K...>)
+{
+    (f(std::integral_constant<int, K>{}), ...);
+}
+// f(integral_constant<int, k>) for k = 0 .. N-1: every k a compile-time
+// constant, so register arrays indexed by k stay in registers
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <int NPC, bool FULL>
+__device__ int lu_wave_reg(double *M, int ld, int S, int &myorig)
+{
+    const int lane = threadIdx.x;
+    const bool row = lane < S;
+    double a[NPC];
+#pragma unroll
+    for (int c = 0; c < NPC; c++) a[c] = (row && (FULL || c < S)) ? M[lane * ld + c] : 0.0;
+    int pos = lane;
+    bool act = row;  // not yet a pivot row
+    int info = 0;
+    static_for<NPC>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (info != 0 || (!FULL && k >= S)) return;
+        const double x = a[k];
+        const double v = act ? (isnan(x) ? ((pos == k) ? INFINITY : -1.0) : fabs(x)) : -2.0;
+        double wv;
+        int wp;
+        wave_argmax(v, act ? pos : 0x7fffffff, wv, wp);
+        const int pl = __builtin_amdgcn_readfirstlane(__builtin_ctzll(__ballot(act & (pos == wp))));
+        const double ukk = readlane_d(x, pl);
+        if (ukk == 0.0) {
+            info = k + 1;
+            return;
+        }
+        const bool isp = lane == pl;
+        pos = isp ? k : ((act & (pos == k)) ? wp : pos);
+        act = act & !isp;
+        const double l = x / ukk;
+        a[k] = act ? l : x;
+        if (act) {
+#pragma unroll
+            for (int c = k + 1; c < NPC; c++)
+                if (FULL || c < S) a[c] = fma(-l, readlane_d(a[c], pl), a[c]);
+        }
+    });
+    if (info != 0) return info;
+    if (row) {
+#pragma unroll
+        for (int c = 0; c < NPC; c++)
+            if (FULL || c < S) M[pos * ld + c] = a[c];
+    }
+    // lane i: the original row now at position i
+    myorig = __builtin_amdgcn_ds_permute(pos * 4, lane);
+    __syncthreads();
+    return 0;
+}
+
 __device__ int lu_wave_any(double *M, int ld, int S, int &myorig)
 {
-    if (S <= 16) return lu_wave<16>(M, ld, S, myorig);
-    if (S <= 32) return lu_wave<32>(M, ld, S, myorig);
-    return lu_wave<64>(M, ld, S, myorig);
+    if (S == 16) return lu_wave_reg<16, true>(M, ld, S, myorig);
+    if (S == 32) return lu_wave_reg<32, true>(M, ld, S, myorig);
+    if (S == 64) return lu_wave_reg<64, true>(M, ld, S, myorig);
+    if (S <= 16) return lu_wave_reg<16, false>(M, ld, S, myorig);
+    if (S <= 32) return lu_wave_reg<32, false>(M, ld, S, myorig);
+    return lu_wave_reg<64, false>(M, ld, S, myorig);
 }
 
 }  // namespace
