@@ -19,6 +19,7 @@ struct Shared {
     double oldP[16][16];
     int iscal[4];
     int rowcnt;
+    double prep[16];  // panel: the representative row of the all-zero rows
 };
 
 // Pointers used by the control group: LDS when the model fits, else global.
@@ -542,8 +543,8 @@ __device__ void panel_factor(T *__restrict__ M, int n, int p, int *piv, int *per
 }
 
 template <int RPT, int NBC, typename T, bool LIST, typename Store>
-__device__ void panel_factor_s(const Store st, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
-                               int *rowcnt, int *info, Shared &sh)
+__device__ void panel_factor_body(const Store st, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
+                                  int *rowcnt, int *info, Shared &sh)
 {
     const int r0 = p * NBC;
     const int nbp = min(NBC, n - r0);
@@ -718,6 +719,197 @@ __device__ void panel_factor_s(const Store st, int n, int p, int *piv, int *perm
     }
     if (tid == 0) *rowcnt = total;
     __syncthreads();
+}
+
+__device__ __forceinline__ bool bits_nonzero(double x) { return __double_as_longlong(x) != 0; }
+__device__ __forceinline__ bool bits_nonzero(float x) { return __float_as_uint(x) != 0u; }
+
+// panel_factor_body with one row per thread for the rows that take part:
+// the C < NT rows listed in sh.ired (every row with an entry other than +0 in
+// the panel's columns, and the rows at the panel's diagonal positions), plus
+// one representative (thread C) of the other rows.  Those rows hold +0 in
+// every panel column; none is a pivot unless a column's largest |a| is 0,
+// which stops the factorization, and each takes the same operations, so
+// the representative's values are every such row's (it is a pivot candidate
+// only where such rows exist, at a position that loses every tie).  Where a
+// non-finite pivot row turns the representative into NaN the rows could
+// differ: nothing is stored and 0 is returned (the caller factors the panel
+// whole).  A zero pivot: piv and info as panel_factor_body leaves them, 2.
+// Otherwise the listed rows are stored, the representative's values are
+// left in sh.prep for the others, 1.
+template <int NBC, typename T, typename Store>
+__device__ int panel_factor_listed(const Store st, int n, int p, int C, int *piv, int *perm_src, int *moved_to,
+                                    int *info, Shared &sh)
+{
+    const int r0 = p * NBC;
+    const int nbp = min(NBC, n - r0);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const bool real = tid < C;
+    const bool repr = tid == C;
+    const int orig = real ? sh.ired[tid] : -1;
+    T a[NBC];
+#pragma unroll
+    for (int c = 0; c < NBC; c++) a[c] = (real && c < nbp) ? st.load(orig, r0 + c) : T(0);
+    int pos = real ? orig : 0x7ffffffe;
+    bool act = real | repr;
+    bool cand = real | (repr & (C < n - r0));
+    int buf = 0;
+    bool stop = false;
+    int stopk = 0;
+    int pk[NBC];
+#pragma unroll
+    for (int j = 0; j < NBC; j++) {
+        const bool live = (j < nbp) & !stop;
+        const int k = r0 + j;
+        const double xv = (double)a[j];
+        const double v = isnan(xv) ? ((pos == k) ? INFINITY : -1.0) : fabs(xv);
+        const double bv = cand ? v : -2.0;
+        const int bp = cand ? pos : 0x7fffffff;
+        double wv;
+        int wp;
+        wave_argmax(bv, bp, wv, wp);
+        if (lane == 0) {
+            sh.cand_v[buf][wave] = wv;
+            sh.cand_p[buf][wave] = wp;
+        }
+        if (cand & (bp == wp)) {
+#pragma unroll
+            for (int c = 0; c < NBC; c++) sh.cand_row[buf][wave][c] = (double)a[c];
+        }
+        __syncthreads();
+        double cv[NWAVES];
+        int cp[NWAVES];
+#pragma unroll
+        for (int ww = 0; ww < NWAVES; ww++) {
+            cv[ww] = sh.cand_v[buf][ww];
+            cp[ww] = sh.cand_p[buf][ww];
+        }
+        int w = 0;
+        double gv = cv[0];
+        int gp = cp[0];
+#pragma unroll
+        for (int ww = 1; ww < NWAVES; ww++) {
+            const bool take = (cv[ww] > gv) | ((cv[ww] == gv) & (cp[ww] < gp));
+            gv = take ? cv[ww] : gv;
+            gp = take ? cp[ww] : gp;
+            w = take ? ww : w;
+        }
+        const double *prow = sh.cand_row[buf][w];
+        const T ukk = (T)prow[j];
+        const bool zero = live & (ukk == T(0));
+        stopk = (zero & !stop) ? k : stopk;
+        stop = stop | zero;
+        const bool go = live & (ukk != T(0));
+        pk[j] = go ? gp : -1;
+        const bool isp = go & act & (pos == gp);
+        const bool isk = go & act & (pos == k);
+        pos = isp ? k : (isk ? gp : pos);
+        act = act & !isp;
+        cand = cand & !isp;
+        T pr[NBC];
+#pragma unroll
+        for (int c = 0; c < NBC; c++) pr[c] = (c > j) ? (T)prow[c] : T(0);
+        const T l = a[j] / ukk;
+        a[j] = (go & act) ? l : a[j];
+        if (go & act) {
+#pragma unroll
+            for (int c = 0; c < NBC; c++)
+                if (c > j) a[c] = fma(-l, pr[c], a[c]);
+        }
+        buf ^= 1;
+    }
+    bool repnan = false;
+    if (repr) {
+#pragma unroll
+        for (int c = 0; c < NBC; c++) {
+            sh.prep[c] = (double)a[c];
+            repnan = repnan | isnan(a[c]);
+        }
+    }
+    if (__syncthreads_or(repnan)) return 0;
+    if (tid == 0) {
+#pragma unroll
+        for (int j = 0; j < NBC; j++)
+            if (pk[j] >= 0) piv[r0 + j] = pk[j];
+        if (stop) st_rlx(info, stopk + 1);
+    }
+    if (stop) return 2;
+    if (real) {
+#pragma unroll
+        for (int c = 0; c < NBC; c++)
+            if (c < nbp) st.store(pos, r0 + c, a[c]);
+        if (!act) perm_src[pos] = orig;
+        if (orig < r0 + nbp) moved_to[orig] = pos;
+    }
+    return 1;
+}
+
+// Panel factorization (panel_factor_body semantics).  With several rows per
+// thread (and no row lists), the rows that take part are gathered first:
+// when fewer than NT of them remain they are factored one per thread
+// (panel_factor_listed), which makes the per-column work that of a panel of
+// at most NT rows; the rows of value 0 take the representative's values.
+template <int RPT, int NBC, typename T, bool LIST, typename Store>
+__device__ void panel_factor_s(const Store st, int n, int p, int *piv, int *perm_src, int *moved_to, int *rows,
+                               int *rowcnt, int *info, Shared &sh)
+{
+    if constexpr (RPT > 1 && !LIST) {
+        const int r0 = p * NBC;
+        const int nbp = min(NBC, n - r0);
+        const int tid = threadIdx.x;
+        const int lane = tid & 63;
+        const int wave = tid >> 6;
+        bool part[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int r = r0 + tid + NT * q;
+            const int rr = min(r, n - 1);  // clamped: loads without branches
+            bool nz = r < r0 + nbp;
+#pragma unroll
+            for (int c = 0; c < NBC; c++) nz = nz | ((c < nbp) & bits_nonzero(st.load(rr, r0 + min(c, nbp - 1))));
+            part[q] = (r < n) & nz;
+            const unsigned long long m = __ballot(part[q]);
+            if (lane == 0) sh.qcount[q][wave] = __popcll(m);
+        }
+        __syncthreads();
+        int total = 0, base[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; q++)
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) {
+                if (w == wave) base[q] = total;
+                total += sh.qcount[q][w];
+            }
+        const int C = __builtin_amdgcn_readfirstlane(total);
+        if (C < NT) {
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                const unsigned long long m = __ballot(part[q]);
+                const int off = __builtin_amdgcn_mbcnt_hi((int)(m >> 32), __builtin_amdgcn_mbcnt_lo((int)m, 0));
+                if (part[q]) sh.ired[base[q] + off] = r0 + tid + NT * q;
+            }
+            __syncthreads();
+            const int done =
+                __builtin_amdgcn_readfirstlane(panel_factor_listed<NBC, T>(st, n, p, C, piv, perm_src, moved_to, info, sh));
+            if (done == 2) return;  // a zero pivot: nothing stored, as panel_factor_body
+            if (done == 1) {
+#pragma unroll
+                for (int q = 0; q < RPT; q++) {
+                    const int r = r0 + tid + NT * q;
+                    if ((r < n) & !part[q]) {
+#pragma unroll
+                        for (int c = 0; c < NBC; c++)
+                            if (c < nbp) st.store(r, r0 + c, (T)sh.prep[c]);
+                    }
+                }
+                __syncthreads();
+                return;
+            }
+        }
+    }
+    panel_factor_body<RPT, NBC, T, LIST>(st, n, p, piv, perm_src, moved_to, rows, rowcnt, info, sh);
 }
 
 // Final row order of factors left by the panels' row moves (perm_src,
