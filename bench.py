@@ -172,6 +172,42 @@ def differentiation_leg(B):
                     "(LASSIE_ERR_NONFINITE, lassie_adjoint_status); FD and oracle-adjoint agreement: GPU tests"}
 
 
+def plain_1024():
+    """One whole solve of the 1024 x 1024 stiff model (k logU[1e-6, 1e6], x0
+    logU[1e-6, 1], seed 0, t in [0, 100], 200 samples) in the plain FP64 form
+    (rtol 1e-6, BDF regrowth, at most 8 RKF rejections in a row): one launch of the single-model kernel, with its
+    phase times and the dense-LU roofline (2/3 S^3 flops per factorization
+    plus 2 S^2 per Newton iteration, against the FP64 peak)."""
+    from lassie_amd import _lib, synth
+    S = 1024
+    m = synth.stiff(S, S, 0, (1e-6, 1e6))
+    t = synth.t_vector(0.0, 100.0, 200)
+    dm = _lib.DeviceModel(m)
+    t0 = time.perf_counter()
+    _, _, st, rc, _ = dm.solve(t, raise_on_maxiter=False, rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
+    wall = time.perf_counter() - t0
+    dm.close()
+    lu = max(st["lu_factorizations"], 1)
+    flops = 2.0 / 3.0 * S ** 3 * st["lu_factorizations"] + 2.0 * S * S * st["newton_iters"]
+    achieved = flops / (st["kernel_ms"] / 1e3) / 1e12
+    res = {"wall_s": wall, "kernel_ms": st["kernel_ms"], "rc": rc, "loop_iters": st["loop_iters"],
+           "bdf_steps": st["bdf_steps"], "newton_iters": st["newton_iters"],
+           "lu_factorizations": st["lu_factorizations"],
+           "phase_ms": {"control_rkf": st["phase_ms"][0], "jacobian": st["phase_ms"][1], "lu": st["phase_ms"][2],
+                        "newton_solves": st["phase_ms"][3]},
+           "lu_ms_per_factorization": st["phase_ms"][2] / lu,
+           "newton_us_per_iteration": 1e3 * st["phase_ms"][3] / max(st["newton_iters"], 1),
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}}
+    traffic = os.path.join(ROOT, "profiles", "traffic_solve_1024.json")
+    if os.path.exists(traffic):
+        with open(traffic) as f:
+            res["roofline"]["traffic"] = json.load(f).get("bytes_per_launch")
+    say(f"  1024x1024 plain solve: {st['kernel_ms'] / 1e3:.2f} s, LU {res['lu_ms_per_factorization']:.3f} ms, "
+        f"Newton iteration {res['newton_us_per_iteration']:.1f} us")
+    return res
+
+
 def precision_form_4096(cap):
     """The FP32-refinement configuration: the 4096 x 4096 stiff model
     (k logU[1e-6, 1e6], x0 logU[1e-6, 1], seed 0, t in [0, 100], 200 samples)
@@ -210,6 +246,20 @@ def precision_form_4096(cap):
     return res
 
 
+def calibrate_copy():
+    """Three copies of a 256 MiB FP64 buffer (each reads and writes 262144 KiB):
+    the known traffic that scales the HBM counters to bytes."""
+    import torch
+    n = 256 * 1024 * 1024 // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    say(f"copied 3 x 256 MiB: {bool(b[123] == a[123])}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,7 +281,15 @@ def main():
     ap.add_argument("--precision-batch", action="store_true",
                     help="also time the batched precision form (FP32 LU + FP64 refinement) on the "
                          "workload and report its error against the chosen form")
+    ap.add_argument("--plain-1024", action="store_true",
+                    help="also run one whole plain FP64 solve of the 1024x1024 stiff model (about 30 s)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy one 256 MiB FP64 buffer into another three times (a kernel of known "
+                         "traffic, to scale the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: instances are independent, "
               "so several GPUs would each run their own share of the batch in separate processes, with no "
@@ -438,6 +496,9 @@ def main():
         dm.close()
     if args.diff_batch > 0:
         line["differentiation_form"] = differentiation_leg(args.diff_batch)
+    if args.plain_1024:
+        say("plain FP64 solve of the 1024x1024 model")
+        line["plain_1024"] = plain_1024()
     if args.precision_4096 > 0:
         say(f"precision form on 4096x4096, {args.precision_4096} loop iterations")
         line["precision_form_4096"] = precision_form_4096(args.precision_4096)
