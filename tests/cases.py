@@ -78,3 +78,18 @@ def hub(partners=20):
         k.append(5.0 + i)
     x0 = [1.0] + [0.5] * partners + [0.0] * partners
     return from_reactions(S, reactions, k, x0)
+
+
+def overflowing_rate(S=300, seed=11):
+    """The stiff generator's S x S model with one bimolecular reaction of a
+    species in the first 16 given k = 1e308 and both reactants 4: its
+    Jacobian entries overflow to inf, so the Newton matrix has infinite
+    entries in the first panel (pivot rows carrying inf, multipliers inf/inf)
+    and every BDF step ends in a NaN repair."""
+    m = synth.stiff(S, S, seed)
+    L, Rm, k, x0 = m.L.copy(), m.Rm.copy(), m.k.copy(), m.x0.copy()
+    r = next(i for i in range(m.R) if L[i].sum() == 2 and np.nonzero(L[i])[0][0] < 16)
+    a, b = np.nonzero(L[r])[0]
+    k[r] = 1e308
+    x0[a] = x0[b] = 4.0
+    return synth.model_from_dense(L, Rm, k, x0)

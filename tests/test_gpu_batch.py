@@ -343,3 +343,24 @@ def test_batch_hub_rows_bit_exact(gpu_lib):
         np.testing.assert_array_equal(out[:n], o_out[:n])
         np.testing.assert_array_equal(y, o_y)
 
+
+def test_batch_overflowing_rate_prefix(gpu_lib):
+    """The workgroup-level kernel on a Newton matrix with infinite entries
+    (cases.overflowing_rate, S = 300): the same panels, repairs and
+    trajectory as the oracle over the first 30 loop iterations."""
+    from tests import cases
+    m = cases.overflowing_rate()
+    t = synth.t_vector(0.0, 100.0, 200)
+    kw = dict(BENCH, max_iters=30, newton_iter=100)
+    print(f"[batch] overflowing rate S={m.S}", flush=True)
+    dm = gpu_lib.DeviceModel(m)
+    outs, ys, stats, _, rc = gpu_lib.solve_batch([dm], t, raise_on_maxiter=False, **kw)
+    dm.close()
+    o_out, o_st, o_rc, _, o_y = O.solve(m, t, **kw)
+    assert o_st["nan_repairs"] >= 10
+    for key in ("loop_iters", "rkf_accepts", "bdf_steps", "newton_iters", "nan_repairs", "lu_failures", "samples"):
+        assert stats[0][key] == o_st[key], (key, stats[0][key], o_st[key])
+    n = int(o_st["samples"])
+    np.testing.assert_array_equal(outs[0][:n], o_out[:n])
+    np.testing.assert_array_equal(ys[0], o_y)
+

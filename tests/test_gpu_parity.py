@@ -113,3 +113,16 @@ def test_eight_column_panels_2100_prefix(gpu_lib):
     g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=4, newton_iter=100, **BENCH)
     assert o[2]["bdf_steps"] >= 2
     assert_identical(g, o)
+
+
+def test_overflowing_rate_prefix(gpu_lib):
+    """Infinite Newton-matrix entries in the first panels (cases.overflowing_rate,
+    S = 300, two rows per thread): the panels that gather their rows meet
+    pivot rows with inf and fall back to the whole panel; every BDF step
+    repairs NaNs.  Both controllers, first 30 loop iterations."""
+    m = cases.overflowing_rate()
+    for opts in (BENCH, FAITHFUL):
+        g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=30, newton_iter=100, **opts)
+        assert o[2]["bdf_steps"] >= 10 and o[2]["nan_repairs"] >= 10
+        assert_identical(g, o)
+
