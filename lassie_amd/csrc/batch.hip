@@ -107,6 +107,24 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
         CtlView v;
         v.recs = I.recs;
         v.hptr = I.hptr;
+        // the right-hand-side records (and their per-species offsets) go to the
+        // LU's staging buffer whenever it is idle (at the start and after each
+        // factorization), so the RKF45 stages and Newton residuals walk them in
+        // LDS; the Jacobian staging and the LU overwrite it in between, when no
+        // right-hand side is evaluated.  The same values in the same order.
+        const int nh_recs = I.hptr[S];
+        const bool recs_fit = (size_t)nh_recs * sizeof(HRec) + sizeof(int) * (size_t)(S + 1) <= sizeof(lp_raw);
+        auto stage_recs = [&]() {
+            if (!recs_fit) return;
+            HRec *lr = reinterpret_cast<HRec *>(lp_raw);
+            int *lh = reinterpret_cast<int *>(lr + nh_recs);
+            for (int e = tid; e < nh_recs; e += NT) lr[e] = I.recs[e];
+            for (int e = tid; e <= S; e += NT) lh[e] = I.hptr[e];
+            v.recs = lr;
+            v.hptr = lh;
+            __syncthreads();
+        };
+        stage_recs();
         v.y1 = s_vec;
         v.ua = s_vec + S;
         v.ub = s_vec + 2 * S;
@@ -262,6 +280,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                 }
                 if (fp64) inf = wg_lu_rm<double, false, RT>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
                 if (!inf) compose_rows<kBatchNB>(perm_src, moved_to, S, s_pf);
+                stage_recs();
                 st_lu++;
                 {
                     const unsigned long long now = wall_clock64();
@@ -294,6 +313,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(BatchArgs A)
                         fp64 = true;
                         st_fb++;
                         inf = wg_lu_rm<double, false, RT>(M, S, piv, perm_src, moved_to, rows, rowcnt, info, sh);
+                        stage_recs();
                         if (inf) continue;  // handled at the top of the loop as an LU failure
                         compose_rows<kBatchNB>(perm_src, moved_to, S, s_pf);
                         for (int s = tid; s < S; s += NT) s_x[s] = rhs[s];
