@@ -211,9 +211,10 @@ def plain_1024():
 def precision_form_4096(cap):
     """The FP32-refinement configuration: the 4096 x 4096 stiff model
     (k logU[1e-6, 1e6], x0 logU[1e-6, 1], seed 0, t in [0, 100], 200 samples)
-    in the plain FP64 form and in the single-model precision form (FP32 grid
+    in the plain FP64 form and in the single-model precision forms (FP32 grid
     LU with 8-column panels, FP64 refinement of every Newton correction to a
-    componentwise backward error with the species' atol as floor), over the
+    componentwise backward error with the species' atol as floor; and the same
+    with the trailing updates on the BF16 matrix cores, 30 sweeps), over the
     first `cap` loop iterations of the same controller; phase times per
     factorization and per Newton iteration, and the error of the final state
     against the FP64 form."""
@@ -224,7 +225,8 @@ def precision_form_4096(cap):
     ctrl = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
     dm = _lib.DeviceModel(m)
     res, finals = {}, {}
-    for name, kw in (("fp64", {}), ("fp32_refine", dict(precision=_lib.PREC_FP32_REFINE))):
+    for name, kw in (("fp64", {}), ("fp32_refine", dict(precision=_lib.PREC_FP32_REFINE)),
+                     ("bf16_refine", dict(precision=_lib.PREC_BF16_REFINE, refine_max_iter=30))):
         say(f"  4096x4096 {name}, first {cap} loop iterations")
         t0 = time.perf_counter()
         _, yf, st, rc, _ = dm.solve(t, raise_on_maxiter=False, max_iters=cap, **ctrl, **kw)
@@ -240,8 +242,10 @@ def precision_form_4096(cap):
         say(f"  {name}: {wall:.1f} s, LU {res[name]['lu_ms_per_factorization']:.2f} ms, Newton iteration "
             f"{res[name]['newton_ms_per_iteration']:.3f} ms")
     dm.close()
-    a, b = finals["fp64"], finals["fp32_refine"]
-    res["max_rel_err_final_state_vs_fp64"] = float(np.max(np.abs(b - a) / (np.abs(a) + m.atol)))
+    a = finals["fp64"]
+    for name in ("fp32_refine", "bf16_refine"):
+        res[name]["max_rel_err_final_state_vs_fp64"] = float(np.max(np.abs(finals[name] - a) / (np.abs(a) + m.atol)))
+    res["max_rel_err_final_state_vs_fp64"] = res["fp32_refine"]["max_rel_err_final_state_vs_fp64"]
     res["loop_iterations"] = cap
     return res
 

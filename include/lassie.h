@@ -51,7 +51,7 @@ extern "C" {
  * per 16 x 16 tile of a 16-column panel, FP32 accumulation). */
 #define LASSIE_PREC_FP64 0          /* plain method: FP64 LU (cuSOLVER getrf/getrs seat) */
 #define LASSIE_PREC_FP32_REFINE 1   /* FP32 LU + FP64 iterative refinement (experimental) */
-#define LASSIE_PREC_BF16_REFINE 2   /* BF16 LU updates + FP64 iterative refinement (experimental, batched only) */
+#define LASSIE_PREC_BF16_REFINE 2   /* BF16 LU updates + FP64 iterative refinement (experimental) */
 /* the whole loop in single precision: `-p float`, Simulation<float>
  * (simulator.cu:259-304); lassie_solve only */
 #define LASSIE_PREC_FP32_LOOP 3
@@ -132,7 +132,8 @@ int lassie_model_destroy(lassie_model *model);
  * factored in FP32 and every Newton solve refined in FP64 (opts.refine_tol,
  * opts.refine_max_iter; a zero FP32 pivot or a refinement that does not
  * converge switches that BDF step to the FP64 LU, counted in
- * stats->refine_fallbacks).  LASSIE_PREC_BF16_REFINE is batched-form only.
+ * stats->refine_fallbacks).  LASSIE_PREC_BF16_REFINE is the same form with the
+ * FP32 factorization's trailing updates on the BF16 matrix cores (experimental).
  * Reference: Simulation<T>::run, Simulation.cuh:230-981; the precision form
  * extends linearSolverLU, Simulation.cuh:1280-1326. */
 int lassie_solve(lassie_model *model, const lassie_opts *opts, const double *t, int32_t nt,

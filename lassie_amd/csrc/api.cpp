@@ -471,11 +471,10 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     else
         lassie_opts_default(&o);
     if (o.precision != LASSIE_PREC_FP64 && o.precision != LASSIE_PREC_FP32_LOOP &&
-        o.precision != LASSIE_PREC_FP32_REFINE)
-        return fail(LASSIE_ERR_UNSUPPORTED,
-                    "single-model solve supports LASSIE_PREC_FP64, LASSIE_PREC_FP32_REFINE and LASSIE_PREC_FP32_LOOP");
+        o.precision != LASSIE_PREC_FP32_REFINE && o.precision != LASSIE_PREC_BF16_REFINE)
+        return fail(LASSIE_ERR_UNSUPPORTED, "unknown precision");
     const int fp32 = o.precision == LASSIE_PREC_FP32_LOOP ? 1 : 0;
-    const int refine = o.precision == LASSIE_PREC_FP32_REFINE ? 1 : 0;
+    const int refine = o.precision == LASSIE_PREC_FP32_REFINE ? 1 : (o.precision == LASSIE_PREC_BF16_REFINE ? 2 : 0);
     if (refine && !(o.refine_tol >= 0.0)) return fail(LASSIE_ERR_INVALID, "refine_tol must be >= 0");
     if (m->S > lassie::kMaxDenseN)
         return fail(LASSIE_ERR_UNSUPPORTED, "dense Newton matrix limited to 4096 species");

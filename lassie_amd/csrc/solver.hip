@@ -68,10 +68,11 @@ __device__ void strip_moves(T *__restrict__ M, int n, int p, int q, const int *p
 // listed by the panel (all others have zero multipliers, so
 // a_ij = fma(-0, u, a_ij) = a_ij), or every row when the U block holds a
 // non-finite value.
-template <int NBC, typename T>
+template <int NBC, typename T, bool BF16 = false>
 __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *perm_src, const int *moved_to,
                              const int *rows, int rowcnt, Shared &sh)
 {
+    static_assert(!BF16 || sizeof(T) == 4, "BF16 updates of FP32 factors");
     const int tid = threadIdx.x;
     const int r0 = p * NBC;
     const int nbp = min(NBC, n - r0);
@@ -121,7 +122,7 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
         }
     }
     // the 8-column update visits every row when U12 holds a non-finite value
-    constexpr bool kMfma = NBC == 16;
+    constexpr bool kMfma = NBC == 16 && !BF16;
     bool all_rows = false;
     if constexpr (kMfma)
         __syncthreads();
@@ -129,7 +130,65 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
         all_rows = __syncthreads_or(nonfinite) != 0;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    if constexpr (kMfma) {
+    if constexpr (BF16) {
+        // every row below the panel on the BF16 matrix cores: one
+        // v_mfma_f32_16x16x16_bf16 per 16 x 16 tile covers the whole panel
+        // (k < 16; an 8-column panel pads k with zeros), the transposed
+        // product of the FP64/FP32 form (A22^T -= U12^T L21^T) with both
+        // operands rounded to BF16 (nearest even) and FP32 accumulation.
+        // Lane l holds A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16], j < 4; D item
+        // i of lane l is D[4(l/16)+i][l%16].
+        typedef short S4 __attribute__((ext_vector_type(4)));
+        typedef float F4 __attribute__((ext_vector_type(4)));
+        auto bits16 = [](float x) { return (short)(__float_as_uint(x) >> 16); };
+        typedef __attribute__((address_space(1))) float GD;
+        GD *Mg = (GD *)M;
+        const size_t last = (size_t)n * n - 1;
+        const size_t sink = (size_t)n * n;
+        const int l16 = lane & 15, lq = lane >> 4;
+        const int rb = r0 + nbp;
+        // 8-column panels list the rows below with a nonzero multiplier: tile
+        // rows come from the list (a zero multiplier row keeps its values
+        // while U12 is finite, as in the vector update); else every row
+        const bool listed = NBC != 16 && !all_rows;
+        const int nrows = listed ? rowcnt : n - rb;
+        const int nt = (nrows + 15) >> 4;
+        S4 ua;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 4 * lq + j;
+            ua[j] = bits16((k < nbp && l16 < ncols) ? to_bf16_rne((float)sh.U12[k][l16]) : 0.f);
+        }
+        for (int t = wave; t < nt; t += NWAVES) {
+            const int idx = 16 * t + l16;
+            const int r = idx < nrows ? (listed ? rows[idx] : rb + idx) : n;
+            const bool okr = r < n;
+            F4 d;
+            S4 bv;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int c = 4 * lq + i;
+                const bool ok = okr && c < ncols;
+                const float v = Mg[ok ? (size_t)(c0 + c) * n + r : last];
+                d[i] = ok ? v : 0.f;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int k = 4 * lq + j;
+                const bool ok = okr && k < nbp;
+                const float v = Mg[ok ? (size_t)(r0 + k) * n + r : last];
+                // padding products are 0 * (-0) = -0, which leaves every value unchanged
+                bv[j] = bits16(ok ? to_bf16_rne(-v) : -0.f);
+            }
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ua, bv, d, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int c = 4 * lq + i;
+                const bool ok = okr && c < ncols;
+                Mg[ok ? (size_t)(c0 + c) * n + r : sink] = d[i];
+            }
+        }
+    } else if constexpr (kMfma) {
         (void)rows;
         (void)rowcnt;
         constexpr int RT = 4;
@@ -431,7 +490,7 @@ __device__ void strip_moves_lds(LDouble *Ls, int ld, int n, int p, int q, const 
 // (look-ahead), so each panel costs one grid barrier; left strips receive the
 // panel's row moves.  On return M holds L\U in final row order.  Returns
 // false on abort.
-template <int RPT, int NBC, typename T>
+template <int RPT, int NBC, typename T, bool BF16 = false>
 __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned int &gen_local,
                                bool &singular)
 {
@@ -488,7 +547,7 @@ __device__ bool lu_factor_grid(const SolveArgs &A, int wg, Shared &sh, unsigned 
                     continue;
                 }
             }
-            strip_update<NBC>(M, n, p, q, A.perm_src, A.moved_to, rows, cnt, sh);
+            strip_update<NBC, T, BF16>(M, n, p, q, A.perm_src, A.moved_to, rows, cnt, sh);
             if (q == p + 1) {
                 __syncthreads();
                 panel_fit<RPT, NBC, T>(MatStore<T, false>{M, n}, n, p + 1, A.piv, A.perm_src, A.moved_to,
@@ -772,9 +831,11 @@ __device__ bool tri_solve_all(const SolveArgs &A, int wg, unsigned int stamp, co
 // the grid; each Newton solve is refined in FP64 (refine_residual); a zero
 // FP32 pivot or a refinement that misses its target within refine_max_iter
 // sweeps switches the step to the FP64 LU of M.
-template <int RPT, int NBC, typename T, bool REF = false>
+template <int RPT, int NBC, typename T, int REF = 0>
 __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
 {
+    // REF 1: FP32 factors; REF 2: FP32 factors with the trailing updates on
+    // the BF16 matrix cores
     static_assert(!REF || sizeof(T) == 8, "the precision form runs the FP64 loop");
     // panel width of the precision form's FP32 factors: NBC, as for the FP64
     // LU.  (16-column FP32 panels above n = 2048 hold 16 rows x 16 columns
@@ -981,7 +1042,7 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 SolveArgs A32 = A;
                 A32.M = reinterpret_cast<double *>(A.M32);
                 A32.lds_strip = 0;
-                if (!lu_factor_grid<RPT, kNbc32, float>(A32, wg, sh, gen_local, singular)) return;
+                if (!lu_factor_grid<RPT, kNbc32, float, REF == 2>(A32, wg, sh, gen_local, singular)) return;
                 fp64 = singular;
                 if (singular) {
                     // zero FP32 pivot: the FP64 LU of M for this step.  Every
@@ -1262,7 +1323,8 @@ template <int RPT, int NBC>
 static const void *kernel_ptr(int fp32, int refine)
 {
     if (fp32) return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, float>);
-    if (refine) return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double, true>);
+    if (refine == 2) return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double, 2>);
+    if (refine) return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double, 1>);
     return reinterpret_cast<const void *>(&solve_kernel<RPT, NBC, double>);
 }
 

@@ -22,6 +22,15 @@ struct Shared {
     double prep[16];  // panel: the representative row of the all-zero rows
 };
 
+__device__ __forceinline__ float to_bf16_rne(float x)
+{
+    // round to the nearest BF16 value (ties to even), kept in an FP32 register
+    unsigned int u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
 // Pointers used by the control group: LDS when the model fits, else global.
 // T is the state type (double; float for the single-precision loop).
 template <typename T>

@@ -8,15 +8,6 @@
 
 namespace lassie {
 
-__device__ __forceinline__ float to_bf16_rne(float x)
-{
-    // round to the nearest BF16 value (ties to even), kept in an FP32 register
-    unsigned int u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return __uint_as_float(u & 0xffff0000u);
-}
-
 // Workgroup-local blocked right-looking LU of a row-major n x n matrix with
 // partial pivoting (panel_factor semantics).  M must be followed by kLuSink
 // writable elements.  RT: row tiles of the trailing update in flight per wave.  Row moves go to every column,

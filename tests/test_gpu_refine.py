@@ -120,7 +120,7 @@ def test_option_checks(gpu_lib):
     dm = gpu_lib.DeviceModel(m)
     t = synth.t_vector(0, 1, 3)
     with pytest.raises(gpu_lib.LassieError):
-        dm.solve(t, precision=gpu_lib.PREC_BF16_REFINE)
+        dm.solve(t, precision=7)
     with pytest.raises(gpu_lib.LassieError):
         dm.solve(t, precision=gpu_lib.PREC_FP32_REFINE, refine_tol=-1.0)
     out, _, st, rc, _ = dm.solve(t, precision=gpu_lib.PREC_FP32_REFINE, **BENCH)
@@ -152,3 +152,79 @@ def test_c3_4096_prefix_matches_oracle_vectors(gpu_lib):
         n = int(st["samples"])
         np.testing.assert_array_equal(out[:n], gold[f"{name}_out"])
     dm.close()
+
+
+# ---- BF16 form (LASSIE_PREC_BF16_REFINE): FP32 factors whose trailing updates
+# run on the BF16 matrix cores (operands rounded to BF16, FP32 accumulation).
+# The oracle has no BF16 restatement, so these tests pin what the form must
+# keep: the FP64 form's results wherever refinement falls back, and the FP64
+# form's decisions and accuracy where the Newton matrices are well
+# conditioned.
+
+def _solve(gpu_lib, m, t, **kw):
+    dm = gpu_lib.DeviceModel(m)
+    out, yf, st, rc, _ = dm.solve(t, raise_on_maxiter=False, **kw)
+    dm.close()
+    return out, yf, st, rc
+
+
+@pytest.mark.parametrize("S", [100, 300])
+def test_bf16_every_step_falls_back(gpu_lib, S):
+    """tol 0 and one sweep: each BDF step runs the BF16-updated FP32 LU, one
+    sweep, then the FP64 LU, so the trajectory equals the FP64 oracle's."""
+    m = synth.stiff(S, 2 * S, 5, (1e-4, 1e4))
+    t = synth.t_vector(0, 2, 5)
+    print(f"[bf16] fallback S={S}", flush=True)
+    out, yf, st, rc = _solve(gpu_lib, m, t, max_iters=400, precision=gpu_lib.PREC_BF16_REFINE, refine_tol=0.0,
+                             refine_max_iter=1, **BENCH)
+    out64, st64, rc64, _, y64 = O.solve(m, t, max_iters=400, **BENCH)
+    assert st["refine_fallbacks"] == st["bdf_steps"] - st["lu_failures"] > 0
+    for key in ("loop_iters", "bdf_steps", "newton_iters", "lu_failures", "nan_repairs", "samples"):
+        assert st[key] == st64[key], key
+    n = int(st64["samples"])
+    np.testing.assert_array_equal(out[:n], out64[:n])
+    np.testing.assert_array_equal(yf, y64)
+
+
+@pytest.mark.parametrize("S,seed,cap", [(48, 3, 0), (300, 7, 40), (513, 7, 30)])
+def test_bf16_mild_steps_close_to_fp64(gpu_lib, S, seed, cap):
+    """BDF steps of 1e-3 on well-conditioned matrices: refinement of the
+    BF16-updated factors meets 1e-14 (floor: the species' atol) without
+    falling back, the controller takes the FP64 form's decisions, and the
+    samples stay within 1e-4 of the FP64 form's, |a - b| / (|b| + atol)."""
+    m = synth.stiff(S, 2 * S, seed, (1e-4, 1e2))
+    t = synth.t_vector(0, 1, 11)
+    kw = dict(max_iters=cap, be_step=1e-3, **BENCH)
+    print(f"[bf16] mild S={S} cap={cap}", flush=True)
+    ref, _, st64, _ = _solve(gpu_lib, m, t, **kw)
+    out, _, st, _ = _solve(gpu_lib, m, t, precision=gpu_lib.PREC_BF16_REFINE, refine_tol=1e-14,
+                           refine_max_iter=30, **kw)
+    assert st["refine_iters"] >= st["newton_iters"] > 0 and st["refine_fallbacks"] == 0
+    for key in ("loop_iters", "bdf_steps", "newton_iters", "nan_repairs", "samples"):
+        assert st[key] == st64[key], key
+    n = int(st["samples"])
+    rel = float((np.abs(out[:n] - ref[:n]) / (np.abs(ref[:n]) + m.atol[None, :])).max()) if n else 0.0
+    print(f"[bf16] S={S} rel={rel:.2e} sweeps={st['refine_iters']} newton={st['newton_iters']}", flush=True)
+    assert rel <= 1e-4
+
+
+def test_bf16_eight_column_panels_2100(gpu_lib):
+    """n > 2048: 8-column FP32 panels; the BF16 update pads k to 16.  First
+    loop iterations of the stiff 2100^2 model (k logU[1e-6, 1e6]) against the
+    FP64 form: the same BDF steps, refinement sweeps run (3 solves fall back
+    to the FP64 LU), and the final state within 2e-2 of the FP64 form's
+    (8.4e-3 measured: BF16's 8-bit mantissa leaves stiff steps' refined
+    corrections further from the FP64 ones than FP32 factors do, and the
+    Newton loop stops at its own tolerance)."""
+    m = synth.stiff(2100, 2100, 0)
+    t = synth.t_vector(0, 100, 200)
+    kw = dict(max_iters=4, newton_iter=100, **BENCH)
+    print("[bf16] 2100 prefix", flush=True)
+    _, y64, st64, _ = _solve(gpu_lib, m, t, **kw)
+    _, yb, st, _ = _solve(gpu_lib, m, t, precision=gpu_lib.PREC_BF16_REFINE, refine_tol=1e-14,
+                          refine_max_iter=30, **kw)
+    assert st["bdf_steps"] == st64["bdf_steps"] >= 2 and st["refine_iters"] > 0
+    rel = float((np.abs(yb - y64) / (np.abs(y64) + m.atol)).max())
+    print(f"[bf16] 2100 rel={rel:.2e} sweeps={st['refine_iters']} fallbacks={st['refine_fallbacks']}", flush=True)
+    assert np.isfinite(yb).all() and rel <= 2e-2
+
