@@ -208,14 +208,19 @@ def plain_1024():
     return res
 
 
-def precision_form_4096(cap):
+PRECISION_FORMS = ("fp64", "fp32_refine", "bf16_refine", "fp32_unrefined", "bf16_unrefined")
+
+
+def precision_form_4096(cap, forms=PRECISION_FORMS):
     """The FP32-refinement configuration: the 4096 x 4096 stiff model
     (k logU[1e-6, 1e6], x0 logU[1e-6, 1], seed 0, t in [0, 100], 200 samples)
     in the plain FP64 form and in the single-model precision forms (FP32 grid
     LU with 8-column panels, FP64 refinement of every Newton correction to a
-    componentwise backward error with the species' atol as floor; and the same
-    with the trailing updates on the BF16 matrix cores, 30 sweeps), over the
-    first `cap` loop iterations of the same controller; phase times per
+    componentwise backward error with the species' atol as floor; the same
+    with the trailing updates on the BF16 matrix cores, 30 sweeps; and both
+    unrefined, refine_max_iter = 0: one FP32 solve per Newton iteration, the
+    modified-Newton iteration's FP64 residual being the only refinement), over
+    the first `cap` loop iterations of the same controller; phase times per
     factorization and per Newton iteration, and the error of the final state
     against the FP64 form."""
     import numpy as np
@@ -225,8 +230,13 @@ def precision_form_4096(cap):
     ctrl = dict(rtol=1e-6, bdf_regrow=1, max_rkf_retries=8)
     dm = _lib.DeviceModel(m)
     res, finals = {}, {}
-    for name, kw in (("fp64", {}), ("fp32_refine", dict(precision=_lib.PREC_FP32_REFINE)),
-                     ("bf16_refine", dict(precision=_lib.PREC_BF16_REFINE, refine_max_iter=30))):
+    table = {"fp64": {}, "fp32_refine": dict(precision=_lib.PREC_FP32_REFINE),
+             "bf16_refine": dict(precision=_lib.PREC_BF16_REFINE, refine_max_iter=30),
+             "fp32_unrefined": dict(precision=_lib.PREC_FP32_REFINE, refine_max_iter=0),
+             "bf16_unrefined": dict(precision=_lib.PREC_BF16_REFINE, refine_max_iter=0)}
+    forms = ["fp64"] + [f for f in forms if f != "fp64"]
+    for name in forms:
+        kw = table[name]
         say(f"  4096x4096 {name}, first {cap} loop iterations")
         t0 = time.perf_counter()
         _, yf, st, rc, _ = dm.solve(t, raise_on_maxiter=False, max_iters=cap, **ctrl, **kw)
@@ -243,9 +253,10 @@ def precision_form_4096(cap):
             f"{res[name]['newton_ms_per_iteration']:.3f} ms")
     dm.close()
     a = finals["fp64"]
-    for name in ("fp32_refine", "bf16_refine"):
+    for name in forms[1:]:
         res[name]["max_rel_err_final_state_vs_fp64"] = float(np.max(np.abs(finals[name] - a) / (np.abs(a) + m.atol)))
-    res["max_rel_err_final_state_vs_fp64"] = res["fp32_refine"]["max_rel_err_final_state_vs_fp64"]
+    if "fp32_refine" in res:
+        res["max_rel_err_final_state_vs_fp64"] = res["fp32_refine"]["max_rel_err_final_state_vs_fp64"]
     res["loop_iterations"] = cap
     return res
 
@@ -282,6 +293,8 @@ def main():
     ap.add_argument("--precision-4096", type=int, default=0, metavar="N",
                     help="also compare the plain and precision forms on the 4096x4096 model over its first N "
                          "loop iterations (not part of the default run)")
+    ap.add_argument("--precision-forms", default=",".join(PRECISION_FORMS),
+                    help="comma-separated forms of --precision-4096 (fp64 always runs, as the reference)")
     ap.add_argument("--precision-batch", action="store_true",
                     help="also time the batched precision form (FP32 LU + FP64 refinement) on the "
                          "workload and report its error against the chosen form")
@@ -505,7 +518,7 @@ def main():
         line["plain_1024"] = plain_1024()
     if args.precision_4096 > 0:
         say(f"precision form on 4096x4096, {args.precision_4096} loop iterations")
-        line["precision_form_4096"] = precision_form_4096(args.precision_4096)
+        line["precision_form_4096"] = precision_form_4096(args.precision_4096, args.precision_forms.split(","))
     line["bench_wall_s"] = elapsed()
     print(json.dumps(line), flush=True)
 

@@ -96,7 +96,9 @@ typedef struct {
     int32_t precision;     /* LASSIE_PREC_*, default LASSIE_PREC_FP64 */
     int64_t max_iters;     /* cap on loop iterations; default 100000000 */
     double refine_tol;     /* refinement stop: |r_i| <= tol*((|M||x|)_i + |b_i| + atol_i) for all i, default 1e-14 */
-    int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback, default 4 */
+    int32_t refine_max_iter; /* refinement sweeps before the FP64 fallback, default 4; 0: one FP32
+                              * solve per Newton iteration, no residual and no fallback (the
+                              * modified-Newton iteration, whose residual is FP64, refines) */
     int32_t reserved;      /* diagnostics: bit 1 keeps the control group's state in global memory */
     int64_t max_newton;    /* extension: cap on the Newton iterations of one solve (one instance of a
                             * batch), checked where max_iters is; 0 (default) = none.  A solve that

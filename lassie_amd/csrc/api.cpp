@@ -604,7 +604,7 @@ extern "C" int lassie_solve(lassie_model *m, const lassie_opts *opts, const doub
     a.lds_strip = lds_strip ? 1 : 0;
     a.strip_ld = strip_ld;
     a.refine = refine;
-    a.refine_max_iter = std::max(1, o.refine_max_iter);
+    a.refine_max_iter = o.refine_max_iter == 0 ? 0 : std::max(1, o.refine_max_iter);
     a.refine_tol = o.refine_tol;
     a.M32 = m->d_M32;
     a.rvec = m->d_rvec;
@@ -812,7 +812,7 @@ static int batch_run(lassie_model *const *models, int32_t n_models, const lassie
     a.max_newton = o.max_newton;
     a.precision = o.precision;
     a.refine_tol = o.refine_tol;
-    a.refine_max_iter = std::max(1, o.refine_max_iter);
+    a.refine_max_iter = o.refine_max_iter == 0 ? 0 : std::max(1, o.refine_max_iter);
     a.s_max = smax;
     a.ws64 = d_ws64;
     a.ws64_stride = ws64_stride;

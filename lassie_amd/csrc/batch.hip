@@ -25,6 +25,8 @@ namespace lassie {
 // components are as accurate as large ones, while components far below atol
 // (under FP32's normal range, which no FP32 correction can reach) do not hold
 // refinement back.  Returns true on convergence; x (double, LDS) holds the result.
+// max_it == 0: the first FP32 solve, x = 0 + (double)d, with no residual and
+// no test (the modified-Newton iteration refines it).
 __device__ bool refine_solve(const double *__restrict__ M64, const float *__restrict__ M32, int n, const int *perm_src,
                              const int *moved_to, const int *pf, const double *atol, const double *rhs, double *x,
                              float *b32, double tol, int max_it, long long &iters, Shared &sh)
@@ -35,6 +37,13 @@ __device__ bool refine_solve(const double *__restrict__ M64, const float *__rest
         b32[i] = (float)rhs[i];
     }
     __syncthreads();
+    if (max_it == 0) {
+        iters++;
+        lu_solve_wg<kBatchNB, float, true>(M32, n, perm_src, moved_to, b32, pf);
+        for (int i = tid; i < n; i += NT) x[i] = x[i] + (double)b32[i];
+        __syncthreads();
+        return true;
+    }
     for (int it = 0; it < max_it; it++) {
         iters++;
         lu_solve_wg<kBatchNB, float, true>(M32, n, perm_src, moved_to, b32, pf);

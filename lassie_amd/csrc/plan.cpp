@@ -235,7 +235,7 @@ extern "C" int lassie_batch_plan_create(lassie_model *const *models, int32_t n_m
     a.max_newton = o.max_newton;
     a.precision = o.precision;
     a.refine_tol = o.refine_tol;
-    a.refine_max_iter = std::max(1, o.refine_max_iter);
+    a.refine_max_iter = o.refine_max_iter == 0 ? 0 : std::max(1, o.refine_max_iter);
     a.s_max = smax;
     a.ws64 = P->d_ws64;
     a.ws64_stride = ws64_stride;

@@ -616,39 +616,47 @@ __device__ __forceinline__ void publish_stamp(unsigned int *flag, unsigned int v
 
 // L y = b for block j (wave 0 of its owner): b = tri[0..S), y -> tri[S..2S).
 // With rsrc, b is read from there instead, in the factors' row order and
-// rounded to T: b_i = (T)rsrc[perm_full[i]].
+// rounded to T: b_i = (T)rsrc[perm_full[i]].  The factors are read with raw
+// buffer loads: the lane's row in the 32-bit vector offset, the column in the
+// scalar offset, and an out-of-range vector offset (which reads zero) for the
+// lanes past the last row, so the loads carry no branches and no 64-bit
+// address arithmetic, and all of a chunk's loads stay in flight together.
 template <typename T>
 __device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp, const double *rsrc = nullptr)
 {
     constexpr int TB = kTriBlock;
-    const int n = A.m.S;
+    constexpr int E = (int)sizeof(T);
+    // wave-uniform (scalar registers: the loads' scalar offsets)
+    const int n = uniform_int(A.m.S);
+    j = uniform_int(j);
     const int lane = threadIdx.x & 63;
     const int R0 = j * TB;
     const int nb = min(TB, n - R0);
     const int i = R0 + lane;
     const bool row = lane < nb;
-    const T *M = reinterpret_cast<const T *>(A.M);
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A.M, n * n * E);
+    const int vo = row ? i * E : kBufOob;
     const T *bp = reinterpret_cast<const T *>(A.tri);
     T *yb = reinterpret_cast<T *>(A.tri) + n;
     unsigned int *fwd = A.tflag;
     // the diagonal block's L entries first: the chain below must not wait on memory
     T d[TB];
 #pragma unroll
-    for (int k = 0; k < TB; k++) d[k] = (k < lane && row) ? M[(size_t)(R0 + k) * n + i] : (T)0.0;
+    for (int k = 0; k < TB; k++) d[k] = BufIO<T>::ld(rs, k < lane ? vo : kBufOob, (R0 + k) * n * E);
     T acc = (T)0.0;
     if (row) acc = rsrc ? (T)rsrc[A.perm_full[i]] : bp[i];
     for (int B = 0; B < j; B++) {
         const int c0 = B * TB;
         T l[16], ln[16];
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) l[kk] = row ? M[(size_t)(c0 + kk) * n + i] : (T)0.0;
+        for (int kk = 0; kk < 16; kk++) l[kk] = BufIO<T>::ld(rs, vo, (c0 + kk) * n * E);
         if (!wait_stamp(fwd + (size_t)B * kFlagStride, stamp, &A.ctl->abort)) return false;
         const T yB = yb[c0 + lane];
 #pragma unroll
         for (int sb = 0; sb < TB / 16; sb++) {
             if (sb + 1 < TB / 16) {
 #pragma unroll
-                for (int kk = 0; kk < 16; kk++) ln[kk] = row ? M[(size_t)(c0 + (sb + 1) * 16 + kk) * n + i] : (T)0.0;
+                for (int kk = 0; kk < 16; kk++) ln[kk] = BufIO<T>::ld(rs, vo, (c0 + (sb + 1) * 16 + kk) * n * E);
             }
 #pragma unroll
             for (int kk = 0; kk < 16; kk++) acc = fma(-l[kk], readlane_d(yB, sb * 16 + kk), acc);
@@ -671,25 +679,35 @@ __device__ bool tri_forward_block(const SolveArgs &A, int j, unsigned int stamp,
     return true;
 }
 
-// U x = y for block j: y = tri[S..2S), x -> tri[2S..3S).
+// U x = y for block j: y = tri[S..2S), x -> tri[2S..3S) (buffer loads as in
+// tri_forward_block).
 template <typename T>
 __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned int stamp)
 {
     constexpr int TB = kTriBlock;
-    const int n = A.m.S;
+    constexpr int E = (int)sizeof(T);
+    // wave-uniform (scalar registers: the loads' scalar offsets)
+    const int n = uniform_int(A.m.S);
+    j = uniform_int(j);
     const int lane = threadIdx.x & 63;
     const int R0 = j * TB;
     const int nb = min(TB, n - R0);
     const int i = R0 + lane;
     const bool row = lane < nb;
-    const T *M = reinterpret_cast<const T *>(A.M);
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A.M, n * n * E);
+    const int vo = row ? i * E : kBufOob;
     const T *yb = reinterpret_cast<const T *>(A.tri) + n;
     T *xb = reinterpret_cast<T *>(A.tri) + 2 * (size_t)n;
+    nblk = uniform_int(nblk);
     unsigned int *bwd = A.tflag + (size_t)nblk * kFlagStride;
     // the diagonal block's U entries (with the diagonal) first
     T d[TB];
 #pragma unroll
-    for (int k = 0; k < TB; k++) d[k] = (k >= lane && k < nb && row) ? M[(size_t)(R0 + k) * n + i] : (T)1.0;
+    for (int k = 0; k < TB; k++) {
+        const bool use = k >= lane && k < nb;
+        const T v = BufIO<T>::ld(rs, use ? vo : kBufOob, (R0 + k) * n * E);
+        d[k] = (use && row) ? v : (T)1.0;
+    }
     T acc = row ? yb[i] : (T)0.0;
     for (int B = nblk - 1; B > j; B--) {
         const int c0 = B * TB;
@@ -698,7 +716,7 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
             const int k = TB - 16 + kk;
-            u[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : (T)0.0;
+            u[kk] = BufIO<T>::ld(rs, k < nbB ? vo : kBufOob, (c0 + k) * n * E);
         }
         if (!wait_stamp(bwd + (size_t)B * kFlagStride, stamp, &A.ctl->abort)) return false;
         const T xB = (lane < nbB) ? xb[c0 + lane] : (T)0.0;
@@ -708,14 +726,15 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) {
                     const int k = (sb - 1) * 16 + kk;
-                    un[kk] = (row && k < nbB) ? M[(size_t)(c0 + k) * n + i] : (T)0.0;
+                    un[kk] = BufIO<T>::ld(rs, k < nbB ? vo : kBufOob, (c0 + k) * n * E);
                 }
             }
+            // columns past the last row (k >= nbB) hold +0 (out-of-range
+            // loads) and xB is 0 there: fma(-0, 0, acc) == acc bit for bit,
+            // so they need no branch (a branch here makes every outstanding
+            // load complete at each join, which would cancel the prefetch)
 #pragma unroll
-            for (int kk = 15; kk >= 0; kk--) {
-                const int k = sb * 16 + kk;
-                if (k < nbB) acc = fma(-u[kk], readlane_d(xB, k), acc);
-            }
+            for (int kk = 15; kk >= 0; kk--) acc = fma(-u[kk], readlane_d(xB, sb * 16 + kk), acc);
             if (sb > 0) {
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) u[kk] = un[kk];
@@ -830,7 +849,10 @@ __device__ bool tri_solve_all(const SolveArgs &A, int wg, unsigned int stamp, co
 // REF: precision form (T = double only): M is rounded to FP32 and factored on
 // the grid; each Newton solve is refined in FP64 (refine_residual); a zero
 // FP32 pivot or a refinement that misses its target within refine_max_iter
-// sweeps switches the step to the FP64 LU of M.
+// sweeps switches the step to the FP64 LU of M.  refine_max_iter == 0: one
+// FP32 solve per Newton iteration, x = 0 + (double)d with no residual (the
+// modified-Newton iteration, whose residual is FP64, refines it), in the
+// plain path's Newton loop (no grid barrier per iteration).
 template <int RPT, int NBC, typename T, int REF = 0>
 __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
 {
@@ -1066,8 +1088,12 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
         // Group 0 forms each residual and updates the iterate; the triangular
         // solves are spread over the owners of the row blocks (tri_*_block).
         bool luFail = false;
+        // REF with refine_max_iter == 0 runs the plain loop below with the
+        // step's factors (FP32, or FP64 after a zero FP32 pivot)
+        bool refined = false;
+        if constexpr (REF) refined = A.refine_max_iter != 0;
         if constexpr (REF) {
-            if (mode == MODE_BDF) {
+            if (mode == MODE_BDF && refined) {
                 // Every group takes part: group 0 forms the residual and updates
                 // the iterate; the FP32 solves, the refinement sweeps and any
                 // FP64 solves are spread over all groups.
@@ -1178,8 +1204,17 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                 }
                 if (wg == 0) st_newton += itN;
             }
-        } else if (mode == MODE_BDF) {
+        }
+        if (mode == MODE_BDF && !refined) {
             const int nblk = (n + kTriBlock - 1) / kTriBlock;
+            // REF (refine_max_iter == 0): the FP32 factors in A32, b = (float)
+            // of the permuted residual, x = 0 + (double)(FP32 solution)
+            bool use32 = false;
+            SolveArgs A32 = A;
+            if constexpr (REF) {
+                use32 = !fp64;
+                A32.M = reinterpret_cast<double *>(A.M32);
+            }
             if (wg == 0) {
                 st_lu++;
                 // final row order of the factors: the panels' row moves composed
@@ -1209,8 +1244,13 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                             st_lufail++;
                             cont = 0;
                         }
-                        if (cont)
-                            for (int s = tid; s < S; s += NT) reinterpret_cast<T *>(A.tri)[s] = v.x[A.perm_full[s]];
+                        if (cont) {
+                            if (use32)
+                                for (int s = tid; s < S; s += NT)
+                                    reinterpret_cast<float *>(A.tri)[s] = (float)v.x[A.perm_full[s]];
+                            else
+                                for (int s = tid; s < S; s += NT) reinterpret_cast<T *>(A.tri)[s] = v.x[A.perm_full[s]];
+                        }
                         __syncthreads();
                         if (tid == 0) {
                             __threadfence();
@@ -1232,24 +1272,38 @@ __global__ void __launch_bounds__(NT) solve_kernel(SolveArgs A)
                     __shared__ int s_ok;
                     if (tid < 64) {
                         bool ok = true;
-                        for (int j = wg; j < nblk && ok; j += G) ok = tri_forward_block<T>(A, j, tri_stamp);
                         int jl = wg;
                         while (jl + G < nblk) jl += G;
-                        for (int j = jl; j >= 0 && ok; j -= G) ok = tri_backward_block<T>(A, j, nblk, tri_stamp);
+                        bool done = false;
+                        if constexpr (REF) {
+                            if (use32) {
+                                for (int j = wg; j < nblk && ok; j += G) ok = tri_forward_block<float>(A32, j, tri_stamp);
+                                for (int j = jl; j >= 0 && ok; j -= G)
+                                    ok = tri_backward_block<float>(A32, j, nblk, tri_stamp);
+                                done = true;
+                            }
+                        }
+                        if (!done) {
+                            for (int j = wg; j < nblk && ok; j += G) ok = tri_forward_block<T>(A, j, tri_stamp);
+                            for (int j = jl; j >= 0 && ok; j -= G) ok = tri_backward_block<T>(A, j, nblk, tri_stamp);
+                        }
                         if (tid == 0) s_ok = ok ? 1 : 0;
                     }
                     __syncthreads();
                     if (!s_ok) return;
+                    if (use32) st_ref++;
                     if (wg == 0) {
                         __threadfence();
                         const T *xs = reinterpret_cast<const T *>(A.tri) + 2 * (size_t)n;
+                        const float *xs32 = reinterpret_cast<const float *>(A.tri) + 2 * (size_t)n;
                         T part = 0.0;
                         for (int s = tid; s < S; s += NT) {
                             const T yv1 = v.yv[s];
                             const T fv = (T)A.m.feed[s];
+                            const T xv = use32 ? (T)(0.0 + (double)xs32[s]) : xs[s];
                             T yv;
                             if (fv == 0)
-                                yv = yv1 + xs[s];
+                                yv = yv1 + xv;
                             else
                                 yv = fv;
                             v.yv[s] = yv;

@@ -53,7 +53,8 @@
  *   refine: the precision form of the Newton linear solve (orc_refine_solve):
  *     FP32 factors of (float)(I - hJ), FP64 iterative refinement, FP64 LU of
  *     the step's matrix when the FP32 LU meets a zero pivot or refinement
- *     does not converge.
+ *     does not converge; refine_max_iter == 0 takes one FP32 solve per
+ *     Newton iteration, unrefined (the Newton iteration refines).
  */
 #include <math.h>
 #include <stdint.h>
@@ -215,7 +216,10 @@ void orc_lu_solve(int n, const double *LU, const int32_t *piv, double *b)
  * error, with the species' absolute tolerance as a floor in the integrator,
  * so that components far below it (1e-40 and less, under FP32's normal
  * range) do not hold refinement back.  x starts at 0 and r at rhs.  Returns
- * 1 on convergence within max_it sweeps, else 0; *iters counts the sweeps. */
+ * 1 on convergence within max_it sweeps, else 0; *iters counts the sweeps.
+ * max_it == 0: the first sweep's x with no residual and no test (returns 1):
+ * the modified-Newton iteration around the solve, whose residual is formed
+ * in FP64, is then the only refinement. */
 int orc_refine_solve(int n, const double *M, const float *LU32, const int32_t *piv, const double *rhs,
                      const double *floor, double *x, double *r, float *b32, double tol, int max_it,
                      int64_t *iters)
@@ -223,6 +227,13 @@ int orc_refine_solve(int n, const double *M, const float *LU32, const int32_t *p
     for (int i = 0; i < n; i++) {
         x[i] = 0.0;
         r[i] = rhs[i];
+    }
+    if (max_it == 0) {
+        (*iters)++;
+        for (int i = 0; i < n; i++) b32[i] = (float)r[i];
+        orc32_lu_solve(n, LU32, piv, b32);
+        for (int i = 0; i < n; i++) x[i] = x[i] + (double)b32[i];
+        return 1;
     }
     for (int it = 0; it < max_it; it++) {
         (*iters)++;

@@ -364,3 +364,33 @@ def test_batch_overflowing_rate_prefix(gpu_lib):
     np.testing.assert_array_equal(outs[0][:n], o_out[:n])
     np.testing.assert_array_equal(ys[0], o_y)
 
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["fp32", "bf16"])
+def test_batch_unrefined_is_refinement_with_an_unbounded_target(gpu_lib, prec):
+    """refine_max_iter = 0 in the batched precision forms (one FP32 solve per
+    Newton iteration, x = 0 + (double)d) equals one refinement sweep whose
+    target every finite correction meets (refine_tol = 1e300), bit for bit,
+    counters included, at sizes on the 16-column LU (48, 200) and the
+    blocked one (512); first 60 loop iterations, BDF steps of 1e-3 on
+    well-conditioned matrices (k logU[1e-4, 1e2]).  (A correction that
+    overflows gives a NaN residual, which the refined form does not accept.)"""
+    models = [synth.stiff(S, 2 * S, 7100 + i, (1e-4, 1e2)) for i, S in enumerate((48, 200, 512))]
+    t = synth.t_vector(0, 1, 11)
+    kw = dict(max_iters=60, be_step=1e-3, precision=prec, **BENCH)
+    print(f"[batch] unrefined precision {prec} sizes {[m.S for m in models]}", flush=True)
+    dms = [gpu_lib.DeviceModel(m) for m in models]
+    a, ya, sa, _, _ = gpu_lib.solve_batch(dms, t, raise_on_maxiter=False, refine_max_iter=0, **kw)
+    b, yb, sb, _, _ = gpu_lib.solve_batch(dms, t, raise_on_maxiter=False, refine_tol=1e300, refine_max_iter=4, **kw)
+    for i, m in enumerate(models):
+        for k in ("loop_iters", "bdf_steps", "newton_iters", "lu_failures", "nan_repairs", "samples",
+                  "refine_iters", "refine_fallbacks"):
+            assert sa[i][k] == sb[i][k], (m.S, k)
+        # one FP32 solve per Newton iteration, except in steps moved to the
+        # FP64 LU by a zero FP32 pivot
+        assert sa[i]["refine_iters"] > 0
+        assert sa[i]["refine_fallbacks"] > 0 or sa[i]["refine_iters"] == sa[i]["newton_iters"]
+        np.testing.assert_array_equal(ya[i], yb[i])
+        np.testing.assert_array_equal(a[i], b[i])
+    for dm in dms:
+        dm.close()

@@ -228,3 +228,57 @@ def test_bf16_eight_column_panels_2100(gpu_lib):
     print(f"[bf16] 2100 rel={rel:.2e} sweeps={st['refine_iters']} fallbacks={st['refine_fallbacks']}", flush=True)
     assert np.isfinite(yb).all() and rel <= 2e-2
 
+
+
+# ---- refine_max_iter = 0: one FP32 (or BF16-updated FP32) solve per Newton
+# iteration, unrefined; the modified-Newton iteration's FP64 residual refines.
+
+@pytest.mark.parametrize("S,seed,cap,kr", [(48, 3, 0, (1e-4, 1e4)), (300, 7, 60, (1e-6, 1e6)),
+                                           (513, 7, 40, (1e-6, 1e6))])
+def test_unrefined_matches_oracle(gpu_lib, S, seed, cap, kr):
+    """Bit-exact against the oracle's refine_max_iter = 0 form (one FP32
+    solve per Newton iteration, x = 0 + (double)d): trajectory, final state
+    and every counter, on stiff models (whole trajectory at 48, capped
+    prefixes with two and four rows per thread at 300 and 513)."""
+    m = synth.stiff(S, 2 * S, seed, kr)
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 50, 100), cap=cap, refine_max_iter=0, **BENCH)
+    assert o[2]["refine_iters"] > 0
+    assert o[2]["refine_fallbacks"] > 0 or o[2]["refine_iters"] == o[2]["newton_iters"]
+    assert_identical(g, o)
+
+
+def test_unrefined_eight_column_panels_2100(gpu_lib):
+    """The 8-column FP32 panels of n > 2048 in the unrefined form, first
+    loop iterations of the stiff 2100^2 model, bit-exact against the oracle."""
+    m = synth.stiff(2100, 2100, 0)
+    g, o = run_both(gpu_lib, m, synth.t_vector(0, 100, 200), cap=4, refine_max_iter=0, newton_iter=100, **BENCH)
+    assert o[2]["bdf_steps"] >= 2 and o[2]["refine_iters"] > 0
+    assert_identical(g, o)
+
+
+@pytest.mark.parametrize("S,seed,cap", [(48, 3, 0), (300, 7, 40), (513, 7, 30)])
+def test_bf16_unrefined_is_refinement_with_an_unbounded_target(gpu_lib, S, seed, cap):
+    """BF16 form on well-conditioned matrices (BDF steps of 1e-3, k logU[1e-4,
+    1e2]): refine_max_iter = 0 equals one refinement sweep whose target every
+    finite correction meets (refine_tol = 1e300) bit for bit, counters
+    included, and the samples stay within 1e-4 of the FP64 form's.  (Where a
+    BF16-updated solve overflows, the sweep's residual is NaN and the refined
+    form goes on to the FP64 LU, while the unrefined one takes the correction
+    and the loop repairs the state: the two then differ by design.)"""
+    m = synth.stiff(S, 2 * S, seed, (1e-4, 1e2))
+    t = synth.t_vector(0, 1, 11)
+    kw = dict(max_iters=cap, be_step=1e-3, **BENCH)
+    print(f"[bf16] unrefined S={S} cap={cap}", flush=True)
+    a, ya, sa, _ = _solve(gpu_lib, m, t, precision=gpu_lib.PREC_BF16_REFINE, refine_max_iter=0, **kw)
+    b, yb, sb, _ = _solve(gpu_lib, m, t, precision=gpu_lib.PREC_BF16_REFINE, refine_tol=1e300,
+                          refine_max_iter=4, **kw)
+    for key in COUNTERS:
+        assert sa[key] == sb[key], key
+    np.testing.assert_array_equal(ya, yb)
+    np.testing.assert_array_equal(a, b)
+    assert sa["refine_iters"] == sa["newton_iters"] > 0 and sa["refine_fallbacks"] == 0
+    ref, _, st64, _ = _solve(gpu_lib, m, t, **kw)
+    n = int(min(sa["samples"], st64["samples"]))
+    rel = float((np.abs(a[:n] - ref[:n]) / (np.abs(ref[:n]) + m.atol[None, :])).max()) if n else 0.0
+    print(f"[bf16] unrefined S={S} rel={rel:.2e} newton={sa['newton_iters']}/{st64['newton_iters']}", flush=True)
+    assert rel <= 1e-4

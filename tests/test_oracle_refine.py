@@ -137,3 +137,40 @@ def test_precision_form_converges_on_mild_steps(S, seed):
         assert sa[k] == sb[k], k
     rel = np.abs(b - a) / (np.abs(a) + 1e-12)
     assert rel.max() <= 1e-5
+
+
+def test_zero_sweeps_is_the_fp32_solve():
+    """max_it = 0: the FP32 solve of (float)rhs alone, no residual and no
+    test, reported as converged after one solve."""
+    rng = np.random.default_rng(7)
+    n = 20
+    M = rng.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    b = rng.uniform(-1, 1, n) * 10.0 ** rng.integers(-8, 3, n)
+    LU32, piv = _factor32(M)
+    x, _, ok, sweeps = O.refine_solve(M, LU32, piv, b, tol=0.0, max_it=0)
+    assert ok and sweeps == 1
+    d = O.lu_solve32(LU32, piv, b.astype(np.float32))
+    np.testing.assert_array_equal(x, d.astype(np.float64))
+    # the FP32 solve alone is about eps_32 * kappa from the FP64 solution,
+    # far from what one refinement sweep reaches
+    rel = np.abs(x - np.linalg.solve(M, b)) / np.abs(np.linalg.solve(M, b))
+    assert 1e-9 < rel.max() < 1e-5
+
+
+@pytest.mark.parametrize("S,seed,tend", [(24, 12, 2.0), (48, 3, 1.0), (100, 1001, 5.0)])
+def test_unrefined_form_is_refinement_with_an_unbounded_target(S, seed, tend):
+    """Whole solves: refine_max_iter = 0 (one FP32 solve per Newton
+    iteration) equals the refined form with a target every first sweep
+    meets (refine_tol = 1e300), bit for bit, counters included: both take
+    x = 0 + (double)(FP32 solve) as the Newton correction."""
+    m = synth.stiff(S, 2 * S, seed, (1e-4, 1e4))
+    t = synth.t_vector(0, tend, 11)
+    a, sa, _, _, ya = O.solve(m, t, refine=1, refine_max_iter=0, **BENCH)
+    b, sb, _, _, yb = O.solve(m, t, refine=1, refine_tol=1e300, refine_max_iter=4, **BENCH)
+    assert sa["bdf_steps"] > 0
+    for k in COUNTERS + ("refine_iters", "refine_fallbacks"):
+        assert sa[k] == sb[k], k
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(ya, yb)
+    # one FP32 solve per Newton iteration; fallbacks only at zero FP32 pivots
+    assert sa["refine_iters"] == sa["newton_iters"] and sa["refine_fallbacks"] == 0
