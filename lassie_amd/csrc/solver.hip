@@ -73,6 +73,9 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
                              const int *rows, int rowcnt, Shared &sh)
 {
     static_assert(!BF16 || sizeof(T) == 4, "BF16 updates of FP32 factors");
+    n = uniform_int(n);
+    p = uniform_int(p);
+    q = uniform_int(q);
     const int tid = threadIdx.x;
     const int r0 = p * NBC;
     const int nbp = min(NBC, n - r0);
@@ -275,39 +278,46 @@ __device__ void strip_update(T *__restrict__ M, int n, int p, int q, const int *
 #pragma unroll
             for (int cc = 0; cc < CPW; cc++) u[kk][cc] = (kk < nbp) ? (T)sh.U12[kk][cbase + cc] : (T)0.0;
         const int nr = all_rows ? n - (r0 + nbp) : rowcnt;
+        // raw buffer accesses: the row in the vector offset (out of range,
+        // reading zero and dropping stores, for masked elements), the column
+        // in the scalar offset; no branches, so every load of a row pair is
+        // in flight at once.  Padding k (kk >= nbp) has l = u = +0, and
+        // fma(-0, 0, acc) == acc bit for bit.
+        constexpr int E = (int)sizeof(T);
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * E);
+        const int cb = uniform_int(cbase);
+        const int nbu = uniform_int(nbp), ncu = uniform_int(ncols);
         for (int x0 = lane; x0 < nr; x0 += 128) {
             const int x1 = x0 + 64;
             const bool has1 = x1 < nr;
             const int i0 = all_rows ? r0 + nbp + x0 : rows[x0];
             const int i1 = has1 ? (all_rows ? r0 + nbp + x1 : rows[x1]) : i0;
+            const int v0 = i0 * E, v1 = has1 ? i1 * E : kBufOob;
             T acc0[CPW], acc1[CPW], l0[NBC], l1[NBC];
 #pragma unroll
             for (int cc = 0; cc < CPW; cc++) {
-                const bool okc = cbase + cc < ncols;
-                acc0[cc] = okc ? M[(size_t)(c0 + cbase + cc) * n + i0] : (T)0.0;
-                acc1[cc] = (okc && has1) ? M[(size_t)(c0 + cbase + cc) * n + i1] : (T)0.0;
+                const bool okc = cb + cc < ncu;
+                acc0[cc] = BufIO<T>::ld(rs, okc ? v0 : kBufOob, (c0 + cb + cc) * n * E);
+                acc1[cc] = BufIO<T>::ld(rs, okc ? v1 : kBufOob, (c0 + cb + cc) * n * E);
             }
 #pragma unroll
             for (int kk = 0; kk < NBC; kk++) {
-                l0[kk] = (kk < nbp) ? M[(size_t)(r0 + kk) * n + i0] : (T)0.0;
-                l1[kk] = (kk < nbp && has1) ? M[(size_t)(r0 + kk) * n + i1] : (T)0.0;
+                l0[kk] = BufIO<T>::ld(rs, kk < nbu ? v0 : kBufOob, (r0 + kk) * n * E);
+                l1[kk] = BufIO<T>::ld(rs, kk < nbu ? v1 : kBufOob, (r0 + kk) * n * E);
             }
 #pragma unroll
             for (int kk = 0; kk < NBC; kk++) {
-                if (kk < nbp) {
 #pragma unroll
-                    for (int cc = 0; cc < CPW; cc++) {
-                        acc0[cc] = fma(-l0[kk], u[kk][cc], acc0[cc]);
-                        acc1[cc] = fma(-l1[kk], u[kk][cc], acc1[cc]);
-                    }
+                for (int cc = 0; cc < CPW; cc++) {
+                    acc0[cc] = fma(-l0[kk], u[kk][cc], acc0[cc]);
+                    acc1[cc] = fma(-l1[kk], u[kk][cc], acc1[cc]);
                 }
             }
 #pragma unroll
             for (int cc = 0; cc < CPW; cc++) {
-                if (cbase + cc < ncols) {
-                    M[(size_t)(c0 + cbase + cc) * n + i0] = acc0[cc];
-                    if (has1) M[(size_t)(c0 + cbase + cc) * n + i1] = acc1[cc];
-                }
+                const bool okc = cb + cc < ncu;
+                BufIO<T>::st(rs, okc ? v0 : kBufOob, acc0[cc], (c0 + cb + cc) * n * E);
+                BufIO<T>::st(rs, okc ? v1 : kBufOob, acc1[cc], (c0 + cb + cc) * n * E);
             }
         }
     }
