@@ -780,11 +780,13 @@ __device__ bool tri_backward_block(const SolveArgs &A, int j, int nblk, unsigned
 __device__ void refine_residual(const SolveArgs &A, int wg, const double *rhs, const double *xo, double *xn,
                                 double *res, unsigned int seq)
 {
-    const int n = A.m.S;
+    const int n = uniform_int(A.m.S);
     const int G = A.G;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const double *Md = A.M;
+    // M by raw buffer loads: the row in the vector offset (out of range for
+    // rows past n, reading zero), the column in the scalar offset
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A.M, n * n * 8);
     const float *d = reinterpret_cast<const float *>(A.tri) + 2 * (size_t)n;
     const double tol = A.refine_tol;
     const int nch = (n + 63) / 64;
@@ -792,6 +794,7 @@ __device__ void refine_residual(const SolveArgs &A, int wg, const double *rhs, c
     for (int c = wg + G * wave; c < nch; c += G * NWAVES) {
         const int i = c * 64 + lane;
         const bool row = i < n;
+        const int vo = row ? i * 8 : kBufOob;
         double pa = 0.0, pm = 0.0;
         for (int j0 = 0; j0 < n; j0 += 64) {
             const int j = j0 + lane;
@@ -804,16 +807,15 @@ __device__ void refine_residual(const SolveArgs &A, int wg, const double *rhs, c
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) {
                     const int k = sb * 16 + kk;
-                    mv[kk] = (row && k < jn) ? Md[(size_t)(j0 + k) * n + i] : 0.0;
+                    mv[kk] = BufIO<double>::ld(rs, k < jn ? vo : kBufOob, uniform_int((j0 + k) * n * 8));
                 }
+                // columns past n: mv = +0 and xj = 0, and pa, pm (sums from
+                // +0) are never -0, so fma(+0, 0, pa) == pa: no branch
 #pragma unroll
                 for (int kk = 0; kk < 16; kk++) {
-                    const int k = sb * 16 + kk;
-                    if (k < jn) {
-                        const double xk = readlane_d(xj, k);
-                        pa = fma(mv[kk], xk, pa);
-                        pm = fma(fabs(mv[kk]), fabs(xk), pm);
-                    }
+                    const double xk = readlane_d(xj, sb * 16 + kk);
+                    pa = fma(mv[kk], xk, pa);
+                    pm = fma(fabs(mv[kk]), fabs(xk), pm);
                 }
             }
         }
