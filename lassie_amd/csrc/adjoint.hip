@@ -24,6 +24,8 @@
 // Jf^T z and (df/dlogk)^T z go through c = H^T z (reaction-major H) and a
 // species-major list of reactant entries, so every sum has one owner thread
 // and a fixed order; there are no atomics.
+#include <type_traits>
+
 #include "adjoint.h"
 #include "batch.h"
 #include "device_common.h"
@@ -135,62 +137,149 @@ __device__ void wg_vjp_jq(const AdjInst &I, const double *y, const double *z, co
 }
 
 // Solve M^T z = b with the factors of wg_lu_rm (P M = L U, rows of M in final
-// order, P the interchanges piv[0..n-1]): U^T w = b forward, L^T u = w
-// backward, in 64-row blocks (diagonal block by one wave with shuffles, the
-// rest by all threads), then z = P^T u (interchanges in reverse order).
-__device__ void lu_solve_t_wg(const double *__restrict__ M, int n, const int *piv, double *b)
+// order): U^T w = b forward, L^T u = w backward, in 64-row blocks (diagonal
+// block by one wave with shuffles, the rest by all threads), then z = P^T u,
+// the scatter z[pf[i]] = u[i] with pf the composed row order of compose_rows
+// (the interchanges applied in reverse order, as one permutation).
+// Arithmetic: every element's updates in the order of a plain column sweep
+// (w_k = b_k / U_kk, then b_i = b_i - U_ki w_k for i > k, k ascending; u
+// likewise with L, k descending inside a block, blocks descending, and the
+// rows outside a block take its columns k ascending).  The factors are read
+// with raw buffer loads (row in the vector offset, column in the scalar
+// offset, an out-of-range offset reading zero where masked) in chunks whose
+// successor is requested before the current one is used; full blocks run
+// without per-column branches.
+__device__ void lu_solve_t_wg(const double *__restrict__ M, int n, const int *pf, double *b)
 {
-    constexpr int TB = 64;
+    constexpr int TB = 64, CD = 16, CO = 8;
+    M = uniform_ptr(M);
+    n = uniform_int(n);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(M, n * n * 8);
     const int NB = (n + TB - 1) / TB;
+    auto ld = [&](int row, int voff) { return BufIO<double>::ld(rs, voff, uniform_int(row * n * 8)); };
+    // the diagonal block of U^T (FWD) or L^T: lane l holds row k0 + kk's
+    // entry in column k0 + l, chunk by chunk
+    auto diag = [&](int k0, int nb, auto fwd, auto full) {
+        constexpr bool FWD = decltype(fwd)::value, FULL = decltype(full)::value;
+        double bi = lane < nb ? b[k0 + lane] : 0.0;
+        const int vo = lane < nb ? (k0 + lane) * 8 : kBufOob;
+        double cur[CD], nxt[CD];
+        auto fetch = [&](double(&dst)[CD], int c) {
+#pragma unroll
+            for (int j = 0; j < CD; j++) {
+                const int kk = FWD ? c * CD + j : TB - 1 - (c * CD + j);
+                dst[j] = ld(k0 + kk, (FULL || kk < nb) ? vo : kBufOob);
+            }
+        };
+        fetch(cur, 0);
+#pragma unroll
+        for (int c = 0; c < TB / CD; c++) {
+            if (c + 1 < TB / CD) fetch(nxt, c + 1);
+#pragma unroll
+            for (int j = 0; j < CD; j++) {
+                const int kk = FWD ? c * CD + j : TB - 1 - (c * CD + j);
+                if (FULL || kk < nb) {
+                    if (FWD) {
+                        const double w = readlane_d(bi, kk) / readlane_d(cur[j], kk);
+                        if (lane == kk) bi = w;
+                        if (lane > kk && lane < nb) bi -= cur[j] * w;
+                    } else {
+                        const double u = readlane_d(bi, kk);
+                        if (lane < kk) bi -= cur[j] * u;
+                    }
+                }
+            }
+            if (c + 1 < TB / CD) {
+#pragma unroll
+                for (int j = 0; j < CD; j++) cur[j] = nxt[j];
+            }
+        }
+        if (lane < nb) b[k0 + lane] = bi;
+    };
+    // rows i in [lo, hi) outside the block: b_i -= sum over the block's
+    // columns k ascending of M[k0 + kk][i] b[k0 + kk], two rows per thread
+    auto outer = [&](int k0, int nb, int lo, int hi, auto full) {
+        constexpr bool FULL = decltype(full)::value;
+        for (int base = lo; base < hi; base += 2 * NT) {
+            const int i0 = base + tid, i1 = i0 + NT;
+            const int v0 = i0 < hi ? i0 * 8 : kBufOob, v1 = i1 < hi ? i1 * 8 : kBufOob;
+            double a0 = i0 < hi ? b[i0] : 0.0, a1 = i1 < hi ? b[i1] : 0.0;
+            double c0[CO], c1[CO], n0[CO], n1[CO];
+            auto fetch = [&](double(&d0)[CO], double(&d1)[CO], int c) {
+#pragma unroll
+                for (int j = 0; j < CO; j++) {
+                    const int kk = c * CO + j;
+                    const bool ok = FULL || kk < nb;
+                    d0[j] = ld(k0 + kk, ok ? v0 : kBufOob);
+                    d1[j] = ld(k0 + kk, ok ? v1 : kBufOob);
+                }
+            };
+            fetch(c0, c1, 0);
+#pragma unroll
+            for (int c = 0; c < TB / CO; c++) {
+                if (c + 1 < TB / CO) fetch(n0, n1, c + 1);
+#pragma unroll
+                for (int j = 0; j < CO; j++) {
+                    const int kk = c * CO + j;
+                    if (FULL || kk < nb) {
+                        const double bk = b[k0 + kk];
+                        a0 -= c0[j] * bk;
+                        a1 -= c1[j] * bk;
+                    }
+                }
+                if (c + 1 < TB / CO) {
+#pragma unroll
+                    for (int j = 0; j < CO; j++) {
+                        c0[j] = n0[j];
+                        c1[j] = n1[j];
+                    }
+                }
+            }
+            if (i0 < hi) b[i0] = a0;
+            if (i1 < hi) b[i1] = a1;
+        }
+    };
     for (int B = 0; B < NB; B++) {
         const int k0 = B * TB, nb = min(TB, n - k0);
         if (tid < 64) {
-            double bi = lane < nb ? b[k0 + lane] : 0.0;
-            for (int kk = 0; kk < nb; kk++) {
-                const double w = readlane_d(bi, kk) / M[(size_t)(k0 + kk) * n + k0 + kk];
-                if (lane == kk) bi = w;
-                if (lane > kk && lane < nb) bi -= M[(size_t)(k0 + kk) * n + k0 + lane] * w;
-            }
-            if (lane < nb) b[k0 + lane] = bi;
+            if (nb == TB)
+                diag(k0, nb, std::true_type{}, std::true_type{});
+            else
+                diag(k0, nb, std::true_type{}, std::false_type{});
         }
         __syncthreads();
-        for (int i = k0 + nb + tid; i < n; i += NT) {
-            double a = b[i];
-            for (int kk = 0; kk < nb; kk++) a -= M[(size_t)(k0 + kk) * n + i] * b[k0 + kk];
-            b[i] = a;
+        if (k0 + nb < n) {
+            if (nb == TB)
+                outer(k0, nb, k0 + nb, n, std::true_type{});
+            else
+                outer(k0, nb, k0 + nb, n, std::false_type{});
         }
         __syncthreads();
     }
     for (int B = NB - 1; B >= 0; B--) {
         const int k0 = B * TB, nb = min(TB, n - k0);
         if (tid < 64) {
-            double bi = lane < nb ? b[k0 + lane] : 0.0;
-            for (int kk = nb - 1; kk >= 0; kk--) {
-                const double u = readlane_d(bi, kk);
-                if (lane < kk) bi -= M[(size_t)(k0 + kk) * n + k0 + lane] * u;
-            }
-            if (lane < nb) b[k0 + lane] = bi;
+            if (nb == TB)
+                diag(k0, nb, std::false_type{}, std::true_type{});
+            else
+                diag(k0, nb, std::false_type{}, std::false_type{});
         }
         __syncthreads();
-        for (int i = tid; i < k0; i += NT) {
-            double a = b[i];
-            for (int kk = 0; kk < nb; kk++) a -= M[(size_t)(k0 + kk) * n + i] * b[k0 + kk];
-            b[i] = a;
+        if (k0 > 0) {
+            if (nb == TB)
+                outer(k0, nb, 0, k0, std::true_type{});
+            else
+                outer(k0, nb, 0, k0, std::false_type{});
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        for (int k = n - 1; k >= 0; k--) {
-            const int p = piv[k];
-            if (p != k) {
-                const double t = b[k];
-                b[k] = b[p];
-                b[p] = t;
-            }
-        }
-    }
+    // z = P^T u: z[pf[i]] = u[i]
+    double u0 = tid < n ? b[tid] : 0.0, u1 = tid + NT < n ? b[tid + NT] : 0.0;
+    __syncthreads();
+    if (tid < n) b[pf[tid]] = u0;
+    if (tid + NT < n) b[pf[tid + NT]] = u1;
     __syncthreads();
 }
 
@@ -450,7 +539,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                             s_z[s] = W.vb[s];
                         }
                         __syncthreads();
-                        lu_solve_t_wg(W.M, S, W.piv, s_z);
+                        lu_solve_t_wg(W.M, S, s_pf, s_z);
                         wg_vjp_jq(I, y, s_z, W.X + (size_t)j * S, dt, W.yb, gk, W.c);
                         for (int s = tid; s < S; s += NT) {
                             W.vb[s] -= s_z[s];
