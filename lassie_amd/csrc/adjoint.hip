@@ -355,6 +355,12 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
     __shared__ Shared sh;
     __shared__ double s_z[kBatchMaxS];
     __shared__ int s_pf[kBatchMaxS];  // row order of the current factors (compose_rows)
+    // operands of the sparse products of a reversed BDF1 step, in LDS: the
+    // step's state, the iterate or correction of the current iteration, the
+    // two cotangent accumulators, and c = H^T z (when R fits)
+    constexpr int kAdjLdsR = 2048;
+    __shared__ double s_y[kBatchMaxS], s_x[kBatchMaxS], s_vb[kBatchMaxS], s_yb[kBatchMaxS];
+    __shared__ double s_c[kAdjLdsR];
     __shared__ int s_idx;
     const int tid = threadIdx.x;
     const int slot = blockIdx.x;
@@ -414,6 +420,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
         m.feed = I.feed;
         m.atol = I.atol;
         const double *fd = I.feed;
+        double *cbuf = R <= kAdjLdsR ? s_c : W.c;
         const double *tdt = A.tape_dt + I.tape_off;
         const int *tkind = A.tape_kind + I.tape_off;
         const int *samp = A.tape_sample + (size_t)inst * A.nt;
@@ -492,7 +499,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                     for (int st = 5; st >= 1; st--) {
                         for (int s = tid; s < S; s += NT) W.ubar[s] = 0.0;
                         __syncthreads();
-                        wg_vjp_f(I, W.u[st], W.kb[st], dt, W.ubar, gk, W.c);
+                        wg_vjp_f(I, W.u[st], W.kb[st], dt, W.ubar, gk, cbuf);
                         for (int s = tid; s < S; s += NT) {
                             if (fd[s] != 0.0) continue;
                             const double ub = W.ubar[s];
@@ -529,26 +536,30 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                             yb = vb;
                             vb = 0.0;
                         }
-                        W.vb[s] = vb;
-                        W.yb[s] = yb;
+                        s_vb[s] = vb;
+                        s_yb[s] = yb;
+                        s_y[s] = y[s];
                     }
                     __syncthreads();
                     for (int j = kind - 1; j >= 0; j--) {
+                        const double *xj = W.X + (size_t)j * S, *vj = W.V + (size_t)j * S;
                         for (int s = tid; s < S; s += NT) {
-                            if (fd[s] != 0.0) W.vb[s] = 0.0;
-                            s_z[s] = W.vb[s];
+                            if (fd[s] != 0.0) s_vb[s] = 0.0;
+                            s_z[s] = s_vb[s];
+                            s_x[s] = xj[s];
                         }
                         __syncthreads();
                         lu_solve_t_wg(W.M, S, s_pf, s_z);
-                        wg_vjp_jq(I, y, s_z, W.X + (size_t)j * S, dt, W.yb, gk, W.c);
+                        wg_vjp_jq(I, s_y, s_z, s_x, dt, s_yb, gk, cbuf);
                         for (int s = tid; s < S; s += NT) {
-                            W.vb[s] -= s_z[s];
-                            W.yb[s] += s_z[s];
+                            s_vb[s] -= s_z[s];
+                            s_yb[s] += s_z[s];
+                            s_x[s] = vj[s];
                         }
                         __syncthreads();
-                        wg_vjp_f(I, W.V + (size_t)j * S, s_z, dt, W.vb, gk, W.c);
+                        wg_vjp_f(I, s_x, s_z, dt, s_vb, gk, cbuf);
                     }
-                    for (int s = tid; s < S; s += NT) W.lam[s] = W.yb[s] + W.vb[s];
+                    for (int s = tid; s < S; s += NT) W.lam[s] = s_yb[s] + s_vb[s];
                     __syncthreads();
                 }
             }
