@@ -131,20 +131,22 @@ def test_option_checks(gpu_lib):
 def test_c3_4096_prefix_matches_oracle_vectors(gpu_lib):
     """The 4096 x 4096 workload (synth.stiff(4096, 4096, 0)), first two loop
     iterations (two BDF1 steps, each with its 4096^2 Jacobian, LU and Newton
-    solves), in the plain FP64 form and in the precision form (FP32 LU, FP64
-    refinement, two refinement fallbacks): state, samples and counters equal
-    the oracle's bit for bit.  The oracle's vectors take about 90 s of CPU, so
-    they come from tests/golden/c3_prefix_4096.npz (make_c3_prefix.py)."""
+    solves), in the plain FP64 form, in the precision form (FP32 LU, FP64
+    refinement, two refinement fallbacks) and in the unrefined precision form
+    (refine_max_iter 0): state, samples and counters equal the oracle's bit
+    for bit.  The oracle's vectors take about two minutes of CPU, so they come
+    from tests/golden/c3_prefix_4096.npz (make_c3_prefix.py)."""
     import os
     gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_prefix_4096.npz"))
     keys = [str(k) for k in gold["keys"]]
     m = synth.stiff(4096, 4096, 0)
     t = synth.t_vector(0, 100, 200)
     dm = gpu_lib.DeviceModel(m)
-    for name, prec in (("fp64", gpu_lib.PREC_FP64), ("fp32_refine", gpu_lib.PREC_FP32_REFINE)):
+    for name, prec, sweeps in (("fp64", gpu_lib.PREC_FP64, 4), ("fp32_refine", gpu_lib.PREC_FP32_REFINE, 4),
+                               ("fp32_unrefined", gpu_lib.PREC_FP32_REFINE, 0)):
         print(f"[refine] 4096x4096 prefix, {name}", flush=True)
         out, yf, st, rc, _ = dm.solve(t, raise_on_maxiter=False, max_iters=2, newton_iter=100, precision=prec,
-                                      refine_tol=1e-14, refine_max_iter=4, **BENCH)
+                                      refine_tol=1e-14, refine_max_iter=sweeps, **BENCH)
         assert (rc == 0) == (int(gold[f"{name}_rc"]) == 0)
         for k, v in zip(keys, gold[f"{name}_stats"]):
             assert st[k] == v, (name, k, st[k], v)
