@@ -40,6 +40,11 @@ struct AdjArgs {
     int *wsi;    // per slot: pivots, row moves, panel row lists, flags
     size_t wsi_stride;
     int *status;  // per instance: 0, or 1 when a replayed LU failed
+    // per slot, a second pair of iterate/correction buffers ((it_max + 1) s_max
+    // doubles each) for the segment step with the most Newton iterations, so
+    // that its reverse step needs no second replay; nullptr: replay it
+    double *vx2;
+    size_t vx2_stride;
 };
 
 hipError_t launch_adjoint(const AdjArgs &args, int slots, hipStream_t stream);

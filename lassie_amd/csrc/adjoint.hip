@@ -447,6 +447,21 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
             const double *ck = A.tape_ckpt + I.ckpt_off + sg * S;
             for (int s = tid; s < S; s += NT) W.seg[s] = ck[s];
             __syncthreads();
+            // the replayed BDF1 step with the most Newton iterations keeps its
+            // iterates in the second buffer pair (tape kinds: -1 RKF45, -2 no
+            // change, >= 0 BDF1 iterations)
+            long long nkeep = -1;
+            AdjWork W2 = W;
+            if (A.vx2) {
+                int best = 0;
+                for (long long n = n0; n < n1 - 1; n++)
+                    if (tkind[n] > best) {
+                        best = tkind[n];
+                        nkeep = n;
+                    }
+                W2.V = A.vx2 + (size_t)slot * A.vx2_stride;
+                W2.X = W2.V + (size_t)(A.it_max + 1) * smax;
+            }
             // states n0 .. n1-1 of this segment
             for (long long n = n0; n < n1 - 1; n++) {
                 const double *y = W.seg + (size_t)(n - n0) * S;
@@ -460,7 +475,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                 } else if (kind == -2) {
                     for (int s = tid; s < S; s += NT) yn[s] = y[s];
                     __syncthreads();
-                } else if (!bdf_iterates(m, I, y, dt, kind, yn, W, s_z, s_pf, sh)) {
+                } else if (!bdf_iterates(m, I, y, dt, kind, yn, n == nkeep ? W2 : W, s_z, s_pf, sh)) {
                     status = 1;
                     break;
                 }
@@ -525,11 +540,14 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                     for (int s = tid; s < S; s += NT) W.lam[s] = W.yb[s];
                     __syncthreads();
                 } else {
-                    if (!bdf_iterates(m, I, y, dt, kind, W.w1, W, s_z, s_pf, sh)) {
+                    // the kept step needs only its factors again (0 iterations)
+                    const bool kept = step == nkeep;
+                    if (!bdf_iterates(m, I, y, dt, kept ? 0 : kind, W.w1, W, s_z, s_pf, sh)) {
                         status = 1;
                         break;
                     }
-                    const double *vm = W.V + (size_t)kind * S;
+                    const AdjWork &Wi = kept ? W2 : W;
+                    const double *vm = Wi.V + (size_t)kind * S;
                     for (int s = tid; s < S; s += NT) {
                         double vb = W.lam[s], yb = 0.0;
                         if (bad_value(vm[s])) {
@@ -542,7 +560,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs A)
                     }
                     __syncthreads();
                     for (int j = kind - 1; j >= 0; j--) {
-                        const double *xj = W.X + (size_t)j * S, *vj = W.V + (size_t)j * S;
+                        const double *xj = Wi.X + (size_t)j * S, *vj = Wi.V + (size_t)j * S;
                         for (int s = tid; s < S; s += NT) {
                             if (fd[s] != 0.0) s_vb[s] = 0.0;
                             s_z[s] = s_vb[s];

@@ -1038,11 +1038,15 @@ extern "C" int lassie_adjoint_batch(lassie_tape *tp, const double *grad_out, dou
                              2 * (size_t)(itmax + 1) * smax + 22 * (size_t)smax + (size_t)rmax;
     const size_t wsi_stride = 3 * (size_t)smax + 16 + Pmax + (size_t)Pmax * smax;
 
+    // optional second iterate/correction pair per slot (the segment's longest
+    // BDF1 step keeps its replayed iterates for its reverse step)
+    const size_t vx2_stride = 2 * (size_t)(itmax + 1) * smax;
+
     lassie::AdjInst *d_inst = nullptr;
     int *d_order = nullptr, *d_next = nullptr, *d_wsi = nullptr, *d_status = nullptr;
-    double *d_gout = nullptr, *d_gk = nullptr, *d_gx = nullptr, *d_ws = nullptr;
+    double *d_gout = nullptr, *d_gk = nullptr, *d_gx = nullptr, *d_ws = nullptr, *d_vx2 = nullptr;
     auto cleanup = [&]() {
-        void *ptrs[] = {d_inst, d_order, d_next, d_wsi, d_status, d_gout, d_gk, d_gx, d_ws};
+        void *ptrs[] = {d_inst, d_order, d_next, d_wsi, d_status, d_gout, d_gk, d_gx, d_ws, d_vx2};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     };
@@ -1058,6 +1062,10 @@ extern "C" int lassie_adjoint_batch(lassie_tape *tp, const double *grad_out, dou
         (e = hipMalloc((void **)&d_ws, sizeof(double) * ws_stride * slots)) != hipSuccess) {
         cleanup();
         return fail(LASSIE_ERR_NOMEM, std::string("adjoint workspace: ") + hipGetErrorString(e));
+    }
+    if (hipMalloc((void **)&d_vx2, sizeof(double) * vx2_stride * slots) != hipSuccess) {
+        (void)hipGetLastError();
+        d_vx2 = nullptr;
     }
     hipStream_t st = m0->stream;
 #define ADJ_TRY(expr)                                                                            \
@@ -1095,6 +1103,8 @@ extern "C" int lassie_adjoint_batch(lassie_tape *tp, const double *grad_out, dou
     a.wsi = d_wsi;
     a.wsi_stride = wsi_stride;
     a.status = d_status;
+    a.vx2 = d_vx2;
+    a.vx2_stride = vx2_stride;
     ADJ_TRY(hipEventRecord(m0->ev0, st));
     ADJ_TRY(lassie::launch_adjoint(a, slots, st));
     ADJ_TRY(hipEventRecord(m0->ev1, st));
